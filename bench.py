@@ -1,0 +1,224 @@
+"""Benchmark of the GPP sigma sum on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload: the canonical mini-app shape number_bands=512, nvband=2,
+ncouls=32768, nodes_per_group=20 (ngpown=1638), seeded random inputs
+(generator config 2), resident in HBM before timing.  One step = one full
+sigma evaluation (prepass, non-occupied kernel, occupied kernel, static
+remainder, fixed-order reduction, 12-double read-back) through the C ABI on
+torch's current stream.  The FP64 path and the mixed path are both timed;
+`value` is the throughput of the fastest path whose achtemp/asxtemp entries
+are all within 1e-8 relative of the FP64 oracle on the checked band slice.
+
+The JSON line also carries:
+  roofline      the non-occupied kernel (the dominant one): algorithmic
+                flops per launch = 25 per non-occupied term, divided by its
+                average HIP-event duration, against the FP64 VALU peak;
+                traffic = HBM bytes per launch measured by rocprofv3 (from
+                profiles/traffic_*.json when present, else null);
+  cpu_baseline  the reference mini-app (oracle/_ref/gppKer_xl, built from
+                gppKerOpenMPFor_xl.cpp) on a band subset of the same shape,
+                all host cores, its own chrono timer; terms/s.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+NB, NV, NC, NPG = 512, 2, 32768, 20
+FLOP_UNOCC, FLOP_OCC = 25.0, 45.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def _load(name, path, pkg=False):
+    import importlib.util
+    if name in sys.modules:
+        return sys.modules[name]
+    kw = {"submodule_search_locations": [str(path.parent)]} if pkg else {}
+    spec = importlib.util.spec_from_file_location(name, path, **kw)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except Exception:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(args):
+    """Reference binary on a band subset of the canonical shape."""
+    exe = ROOT / "oracle" / "_ref" / "gppKer_xl"
+    cores = host_cores()
+    ng = NC // NPG
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread",
+               OMP_PLACES="cores")
+    if exe.exists():
+        def run(nb):
+            out = subprocess.run([str(exe), str(nb), str(NV), str(NC), str(NPG)],
+                                 env=env, capture_output=True, text=True,
+                                 timeout=300, check=True).stdout
+            t = float(re.search(r"Chrono Time Taken \*+= ([0-9.eE+-]+)", out).group(1))
+            return t, out
+        t4, _ = run(4)
+        rate = 4 * ng * NC * 3 / t4
+        nb = int(max(4, min(NB, round(args.cpu_seconds * rate / (ng * NC * 3)))))
+        t, out = run(nb)
+        terms = nb * ng * NC * 3
+        return {"value": terms / t, "unit": "terms/s", "cores": cores,
+                "kind": "reference",
+                "sample": f"reference gppKerOpenMPFor_xl.cpp binary, {nb} {NV} {NC} {NPG} "
+                          f"(band subset of 512 {NV} {NC} {NPG}; its constant fills), "
+                          f"OMP_NUM_THREADS={cores}, chrono {t:.2f} s"}
+    # Port fallback: the oracle (compensated sums, so slower than the reference).
+    gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
+    orc = _load("gpp_oracle", ROOT / "oracle" / "oracle.py")
+    inp = gppk.fill_inputs(2, 3, NC, NPG, nvband=NV, seed=1)
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    t0 = time.perf_counter()
+    orc.sigma(inp, want_achstemp=False)
+    t = time.perf_counter() - t0
+    return {"value": 3 * ng * NC * 3 / t, "unit": "terms/s", "cores": cores,
+            "kind": "port", "sample": f"CPU oracle, 3 bands of 512 {NV} {NC} {NPG}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--precisions", default="fp64,mixed")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sum would shard over G' columns with one all-reduce of 12 "
+              "doubles); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
+    ng = NC // NPG
+    log(f"generating inputs {NB} {NV} {NC} {NPG} seed {args.seed}")
+    inp = gppk.fill_inputs(2, NB, NC, NPG, nvband=NV, seed=args.seed)
+    dev = {k: (torch.from_numpy(np.ascontiguousarray(v)).cuda() if isinstance(v, np.ndarray) else v)
+           for k, v in inp.items()}
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    h = gppk.Handle(0)
+    terms = NB * ng * NC * 3
+    unocc_terms = (NB - NV) * ng * NC * 3
+    occ_terms = NV * ng * NC * 3
+
+    runs = {}
+    for prec in args.precisions.split(","):
+        call = lambda: h.sigma(dev, precision=prec, memory=gppk.MEM_DEVICE,
+                               stream=stream.cuda_stream)
+        for _ in range(args.warmup):
+            call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = []
+        for i in range(args.steps):
+            res.append(call())
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        ku = float(np.mean([r["unocc_kernel_ms"] for r in res]))
+        ko = float(np.mean([r["occ_kernel_ms"] for r in res]))
+        runs[prec] = {"ms_per_step": dt * 1e3, "terms_per_s": terms / dt,
+                      "unocc_kernel_ms": ku, "occ_kernel_ms": ko,
+                      "achtemp": res[-1]["achtemp"], "asxtemp": res[-1]["asxtemp"],
+                      "fallback": res[-1]["fp64_fallback_terms"]}
+        log(f"{prec}: {dt*1e3:.3f} ms/step, {terms/dt:.4e} terms/s, "
+            f"unocc kernel {ku:.3f} ms, occ kernel {ko:.3f} ms")
+
+    # Error of each path against the FP64 oracle on a band slice (bands 0..3:
+    # both occupied bands and one unoccupied band at the full G-vector shape),
+    # and of mixed against FP64 on the full problem.
+    errs = {}
+    if not args.no_check:
+        orc = _load("gpp_oracle", ROOT / "oracle" / "oracle.py")
+        sub = gppk.band_slice(inp, 0, 3)
+        ref = orc.sigma(sub, want_achstemp=False)
+        for prec in runs:
+            g = h.sigma(sub, precision=prec)
+            e = max(float(np.max(np.abs(g[k] - ref[k]) / np.abs(ref[k])))
+                    for k in ("achtemp", "asxtemp"))
+            errs[prec] = e
+            log(f"{prec}: max rel err vs oracle (bands 0-2) {e:.3e}")
+        if "fp64" in runs and "mixed" in runs:
+            e = max(float(np.max(np.abs(runs["mixed"][k] - runs["fp64"][k]) /
+                                 np.abs(runs["fp64"][k]))) for k in ("achtemp", "asxtemp"))
+            errs["mixed_vs_fp64_full"] = e
+            log(f"mixed vs fp64 full problem: {e:.3e}")
+
+    ok = [p for p in runs if errs.get(p, 0.0) <= 1e-8 and
+          (p != "mixed" or errs.get("mixed_vs_fp64_full", 0.0) <= 1e-8)]
+    best = max(ok, key=lambda p: runs[p]["terms_per_s"]) if ok else "fp64"
+    b = runs[best]
+    ku = b["unocc_kernel_ms"] * 1e-3
+    achieved = FLOP_UNOCC * unocc_terms / ku / 1e12
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_unocc_{best}_512_2_32768_20.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+    line = {
+        "metric": "GPP sigma terms/s on 1 MI355X, mixed vs FP64 @<=1e-8 rel err; % VALU peak",
+        "value": b["terms_per_s"],
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": b["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "mixed(f32 terms, f64 sums)",
+        "data": "synthetic",
+        "config": {"workload": "GPP sigma sum, canonical gppKer shape, seeded random inputs",
+                   "number_bands": NB, "nvband": NV, "ncouls": NC,
+                   "nodes_per_group": NPG, "ngpown": ng, "terms": terms,
+                   "path": best},
+        "roofline": {"bound": "valu", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "kernel": f"unocc_{best}",
+                     "flops_per_launch": FLOP_UNOCC * unocc_terms,
+                     "avg_launch_ms": b["unocc_kernel_ms"]},
+        "paths": {p: {"terms_per_s": r["terms_per_s"], "ms_per_step": r["ms_per_step"],
+                      "unocc_kernel_ms": r["unocc_kernel_ms"],
+                      "occ_kernel_ms": r["occ_kernel_ms"],
+                      "valu_frac_fp64_peak": FLOP_UNOCC * unocc_terms / (r["unocc_kernel_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                      "max_rel_err_vs_oracle": errs.get(p)} for p, r in runs.items()},
+        "mixed_vs_fp64_full_rel_err": errs.get("mixed_vs_fp64_full"),
+    }
+    if "fp64" in runs and "mixed" in runs:
+        line["mixed_speedup_vs_fp64"] = runs["mixed"]["terms_per_s"] / runs["fp64"]["terms_per_s"]
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        line["cpu_baseline"] = cpu_baseline(args)
+    h.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

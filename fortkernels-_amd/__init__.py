@@ -1,0 +1,272 @@
+"""Python host binding of the MI355X GPP sigma library (lib/libgpp.so).
+
+The compute path is the C ABI declared in include/gpp_sigma.h; this module
+only wraps it with ctypes so that tests, the benchmark and Python callers
+can hand it numpy arrays (host memory) or torch device tensors (device
+memory).  It never computes sigma itself: if the shared library is missing
+the loader raises, there is no fallback.
+
+Reference interface mirrored: the gppKer loop nest (gppKer.cpp:270-361) and
+the operator split tillNvband_kernel / achtemp_kernel
+(gppKer_gpuComplex.cpp:127,148).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libgpp.so"
+CLI_PATH = PKG_DIR / "bin" / "gpp"
+
+GPP_OK = 0
+GPP_EINVAL = -1
+GPP_ENOMEM = -2
+GPP_EDEVICE = -3
+GPP_ENONFINITE = -4
+GPP_EUNSUPPORTED = -5
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+SEM_GPPKER = 0
+SEM_GPPKER_STRICT = 1
+SEM_BGW = 2
+MEM_HOST = 0
+MEM_DEVICE = 1
+FLAG_ALL_BANDS_UNOCC = 1
+FLAG_SKIP_UNOCC = 2
+FLAG_CHECK_FINITE = 4
+FLAG_NO_ACHSTEMP = 8
+
+SEMANTICS = {"gppker": SEM_GPPKER, "strict": SEM_GPPKER_STRICT, "bgw": SEM_BGW}
+PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
+
+# Exported symbols, one per declaration of include/gpp_sigma.h.
+EXPORTS = (
+    "gpp_create", "gpp_destroy", "gpp_last_error", "gpp_default_options",
+    "gpp_sigma_ex", "gpp_sigma_batch", "gpp_sigma", "gpp_achtemp_kernel",
+    "gpp_tillNvband_kernel", "gpp_fill_inputs", "gpp_version",
+)
+
+
+class c128(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("number_bands", C.c_int), ("nvband", C.c_int),
+        ("ncouls", C.c_int), ("ngpown", C.c_int),
+        ("aqsmtemp", C.c_void_p), ("aqsntemp", C.c_void_p),
+        ("I_eps_array", C.c_void_p), ("wtilde_array", C.c_void_p),
+        ("vcoul", C.c_void_p), ("inv_igp_index", C.c_void_p),
+        ("indinv", C.c_void_p), ("wx_array", C.c_void_p),
+        ("wx_per_band", C.c_int),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int), ("semantics", C.c_int),
+        ("memory", C.c_int), ("flags", C.c_int),
+        ("stream", C.c_void_p), ("resonance_threshold", C.c_double),
+        ("tile_ig", C.c_int), ("waves_per_block", C.c_int),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("achtemp", c128 * 3), ("asxtemp", c128 * 3), ("achstemp", c128),
+        ("acht_n1_loc", C.c_void_p),
+        ("unocc_kernel_ms", C.c_double), ("occ_kernel_ms", C.c_double),
+        ("total_ms", C.c_double), ("terms", C.c_double),
+        ("fp64_fallback_terms", C.c_longlong),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
+    """Load libgpp.so (built in-tree by __graft_entry__.build()).  Raises
+    OSError when it is missing: there is no CPU fallback."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise OSError(f"GPP library not built: {p} (run __graft_entry__.build())")
+    lib = C.CDLL(str(p))
+    vp, ip = C.c_void_p, C.c_int
+    lib.gpp_create.argtypes = [C.POINTER(vp), ip]
+    lib.gpp_destroy.argtypes = [vp]
+    lib.gpp_last_error.argtypes = [vp]
+    lib.gpp_last_error.restype = C.c_char_p
+    lib.gpp_default_options.argtypes = [C.POINTER(Options)]
+    lib.gpp_sigma_ex.argtypes = [vp, C.POINTER(Problem), C.POINTER(Options),
+                                 C.POINTER(Result)]
+    lib.gpp_sigma_batch.argtypes = [vp, C.POINTER(Problem), ip,
+                                    C.POINTER(Options), C.POINTER(Result)]
+    lib.gpp_sigma.argtypes = [ip, ip, ip, ip] + [vp] * 10
+    lib.gpp_achtemp_kernel.argtypes = [ip, ip] + [vp] * 8 + [ip] + [vp] * 6
+    lib.gpp_tillNvband_kernel.argtypes = ([vp] * 8 + [ip, ip, ip, vp, ip, ip,
+                                          C.c_double, ip])
+    lib.gpp_fill_inputs.argtypes = [ip, C.c_ulonglong, ip, ip, ip, ip] + [vp] * 8
+    lib.gpp_version.restype = C.c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class GppError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gpp error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a) -> int | None:
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return a.data_ptr()
+    raise TypeError(f"unsupported array type {type(a)}")
+
+
+def make_problem(inp: dict) -> Problem:
+    """Problem struct from a dict of arrays (numpy host or torch device).
+    Keys: aqsmtemp, aqsntemp, I_eps_array, wtilde_array (complex128, or
+    float64 with a trailing re/im axis), vcoul (float64), inv_igp_index,
+    indinv (int32), wx_array (float64, [3] or [NB,3]), nvband, and
+    optionally number_bands/ncouls/ngpown."""
+    aqsm = inp["aqsmtemp"]
+    ieps = inp["I_eps_array"]
+    nb = inp.get("number_bands", aqsm.shape[0])
+    nc = inp.get("ncouls", aqsm.shape[1] if aqsm.ndim >= 2 else inp["vcoul"].shape[0])
+    ng = inp.get("ngpown", ieps.shape[0])
+    wx = inp["wx_array"]
+    per_band = int(wx.ndim == 2 or wx.shape[0] != 3)
+    return Problem(nb, int(inp["nvband"]), nc, ng,
+                   _ptr(aqsm), _ptr(inp["aqsntemp"]), _ptr(ieps),
+                   _ptr(inp["wtilde_array"]), _ptr(inp["vcoul"]),
+                   _ptr(inp["inv_igp_index"]), _ptr(inp["indinv"]),
+                   _ptr(wx), per_band)
+
+
+def _cplx(z: c128) -> complex:
+    return complex(z.re, z.im)
+
+
+class Handle:
+    """Owns one gpp_handle (device buffers, stream, events)."""
+
+    def __init__(self, device: int = -1, lib: C.CDLL | None = None):
+        self.lib = lib or load_library()
+        h = C.c_void_p()
+        rc = self.lib.gpp_create(C.byref(h), device)
+        if rc != GPP_OK:
+            raise GppError(rc, "gpp_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gpp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def last_error(self) -> str:
+        return self.lib.gpp_last_error(self.h).decode()
+
+    def sigma(self, inp: dict, precision="fp64", semantics="gppker",
+              memory=None, flags=0, stream=None, resonance_threshold=0.0,
+              tile_ig=0, waves_per_block=0) -> dict:
+        prob = make_problem(inp)
+        if memory is None:
+            memory = MEM_DEVICE if hasattr(inp["aqsmtemp"], "data_ptr") else MEM_HOST
+        opt = Options()
+        self.lib.gpp_default_options(C.byref(opt))
+        opt.precision = PRECISION.get(precision, precision)
+        opt.semantics = SEMANTICS.get(semantics, semantics)
+        opt.memory = memory
+        opt.flags = flags
+        opt.stream = stream
+        opt.resonance_threshold = resonance_threshold
+        opt.tile_ig = tile_ig
+        opt.waves_per_block = waves_per_block
+        res = Result()
+        rc = self.lib.gpp_sigma_ex(self.h, C.byref(prob), C.byref(opt), C.byref(res))
+        if rc != GPP_OK:
+            raise GppError(rc, self.last_error())
+        return {
+            "achtemp": np.array([_cplx(z) for z in res.achtemp]),
+            "asxtemp": np.array([_cplx(z) for z in res.asxtemp]),
+            "achstemp": _cplx(res.achstemp),
+            "unocc_kernel_ms": res.unocc_kernel_ms,
+            "occ_kernel_ms": res.occ_kernel_ms,
+            "total_ms": res.total_ms,
+            "terms": res.terms,
+            "fp64_fallback_terms": res.fp64_fallback_terms,
+        }
+
+
+def fill_inputs(config: int, number_bands: int, ncouls: int,
+                nodes_per_group: int, nvband: int, seed: int = 12345,
+                lib: C.CDLL | None = None) -> dict:
+    """Synthetic inputs (numpy, host) from the library's generator
+    (gpp_fill_inputs): config 1 constant fills, 2 canonical random,
+    3 band-resolved energies with near-resonant wtilde."""
+    lib = lib or load_library()
+    nb, nc = number_bands, ncouls
+    ng = nc // nodes_per_group
+    d = {
+        "aqsmtemp": np.empty((nb, nc), np.complex128),
+        "aqsntemp": np.empty((nb, nc), np.complex128),
+        "I_eps_array": np.empty((ng, nc), np.complex128),
+        "wtilde_array": np.empty((ng, nc), np.complex128),
+        "vcoul": np.empty(nc, np.float64),
+        "inv_igp_index": np.empty(max(ng, 1), np.int32),
+        "indinv": np.empty(nc, np.int32),
+        "wx_array": np.empty((nb, 3) if config == 3 else 3, np.float64),
+    }
+    rc = lib.gpp_fill_inputs(config, seed, nb, nc, ng, nodes_per_group,
+                             *[_ptr(d[k]) for k in (
+                                 "aqsmtemp", "aqsntemp", "I_eps_array",
+                                 "wtilde_array", "vcoul", "inv_igp_index",
+                                 "indinv", "wx_array")])
+    if rc != GPP_OK:
+        raise GppError(rc, "gpp_fill_inputs failed")
+    d["inv_igp_index"] = d["inv_igp_index"][:ng].copy()
+    d.update(number_bands=nb, ncouls=nc, ngpown=ng, nvband=nvband)
+    return d
+
+
+def band_slice(inp: dict, n1_begin: int, n1_end: int) -> dict:
+    """The sub-problem of bands [n1_begin, n1_end) (band additivity: the full
+    sums are the sums of the sub-problems')."""
+    d = dict(inp)
+    d["aqsmtemp"] = inp["aqsmtemp"][n1_begin:n1_end]
+    d["aqsntemp"] = inp["aqsntemp"][n1_begin:n1_end]
+    if inp["wx_array"].ndim == 2:
+        d["wx_array"] = inp["wx_array"][n1_begin:n1_end]
+    d["number_bands"] = n1_end - n1_begin
+    d["nvband"] = max(0, min(inp["nvband"], n1_end) - n1_begin)
+    return d

@@ -1,0 +1,172 @@
+// gpp_inputs.cpp — synthetic inputs for the GPP sigma sum, host memory,
+// reference layout.  Counter-based: every element is a pure function of
+// (seed, stream, index), so any sub-range can be generated independently
+// (threads, or a band subset) and the result does not depend on the thread
+// count.
+//
+// config 1: the constant fills of gppKer.cpp:241-264 (0.5+0.5i everywhere,
+//           vcoul = 1, inv_igp_index[k] = (k+1)*NC/NG, identity indinv,
+//           wx = e_lk - e_n1kq + dw*(iw-1) = {3,4,5}, gppKer.cpp:276-280).
+// config 2: seeded random inputs of the canonical 512/2/32768/20 shape:
+//           aqsm, aqsn ~ N(0,1)/sqrt(NC) per component; I_eps re ~ U(0,1),
+//           im ~ U(-0.1,0.1); wtilde re ~ U(0.5,3), im ~ U(-0.05,0.05);
+//           vcoul[ig] = 1/(0.01 + (ig/NC)^(2/3)); inv_igp_index[i] =
+//           i*nodes_per_group; identity indinv; wx = {3,4,5}.
+// config 3: config 2 plus band-resolved energies e_n1 = sorted U(-20,20),
+//           wx[n1][iw] = 4 - e_n1 + (iw-1), and 5% of the wtilde real parts
+//           moved to within 1e-4 of a randomly chosen wx[n1][iw].
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <thread>
+#include <vector>
+
+#include "../../include/gpp_sigma.h"
+
+namespace {
+
+enum Stream : uint64_t {
+  kAqsm = 1,
+  kAqsn = 2,
+  kIeps = 3,
+  kWtilde = 4,
+  kEnergy = 5,
+  kResPick = 6,
+  kResWhere = 7,
+};
+
+inline uint64_t mix64(uint64_t z) {
+  // SplitMix64 finaliser.
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+inline uint64_t draw(uint64_t seed, uint64_t stream, uint64_t i) {
+  const uint64_t key = mix64(seed * 0xD1B54A32D192ED03ull + stream);
+  return mix64(key ^ mix64(i));
+}
+
+// [0,1) with 53 random bits.
+inline double u01(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }
+
+// Standard normal pair by Box-Muller from draws 2i and 2i+1.
+inline void normal2(uint64_t seed, uint64_t stream, uint64_t i, double *a,
+                    double *b) {
+  const double u1 = ((draw(seed, stream, 2 * i) >> 11) + 1) * 0x1.0p-53;
+  const double u2 = u01(draw(seed, stream, 2 * i + 1));
+  const double r = std::sqrt(-2.0 * std::log(u1));
+  const double t = 6.283185307179586 * u2;
+  *a = r * std::cos(t);
+  *b = r * std::sin(t);
+}
+
+template <class F>
+void parallel_for(size_t n, F f) {
+  unsigned nt = std::thread::hardware_concurrency();
+  if (nt == 0) nt = 1;
+  if (nt > 16) nt = 16;
+  if (n < (size_t)1 << 16) nt = 1;
+  if (nt == 1) {
+    f(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t chunk = (n + nt - 1) / nt;
+  for (unsigned t = 0; t < nt; ++t) {
+    const size_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    th.emplace_back([=] { f(b, e); });
+  }
+  for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" int gpp_fill_inputs(int config, unsigned long long seed,
+                               int number_bands, int ncouls, int ngpown,
+                               int nodes_per_group, gpp_c128 *aqsmtemp,
+                               gpp_c128 *aqsntemp, gpp_c128 *I_eps_array,
+                               gpp_c128 *wtilde_array, double *vcoul,
+                               int *inv_igp_index, int *indinv,
+                               double *wx_array) {
+  const int NB = number_bands, NC = ncouls, NG = ngpown;
+  if (config < 1 || config > 3 || NB < 0 || NC <= 0 || NG < 0)
+    return GPP_EINVAL;
+  if (!aqsmtemp || !aqsntemp || !I_eps_array || !wtilde_array || !vcoul ||
+      !inv_igp_index || !indinv || !wx_array)
+    return GPP_EINVAL;
+  const size_t nbnc = (size_t)NB * NC, ngnc = (size_t)NG * NC;
+
+  for (int i = 0; i < NC; ++i) indinv[i] = i;
+
+  if (config == 1) {
+    const gpp_c128 c = {0.5, 0.5};
+    parallel_for(nbnc, [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) aqsmtemp[i] = aqsntemp[i] = c;
+    });
+    parallel_for(ngnc, [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) I_eps_array[i] = wtilde_array[i] = c;
+    });
+    for (int i = 0; i < NC; ++i) vcoul[i] = 1.0;
+    for (int k = 0; k < NG; ++k)
+      inv_igp_index[k] = (int)((long long)(k + 1) * NC / NG);
+    for (int iw = 0; iw < 3; ++iw) wx_array[iw] = 10.0 - 6.0 + (iw - 1);
+    return GPP_OK;
+  }
+
+  if (nodes_per_group <= 0 ||
+      (long long)(NG > 0 ? NG - 1 : 0) * nodes_per_group >= NC)
+    return GPP_EINVAL;
+  const double inv_sqrt_nc = 1.0 / std::sqrt((double)NC);
+  parallel_for(nbnc, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      double x, y;
+      normal2(seed, kAqsm, i, &x, &y);
+      aqsmtemp[i] = {x * inv_sqrt_nc, y * inv_sqrt_nc};
+      normal2(seed, kAqsn, i, &x, &y);
+      aqsntemp[i] = {x * inv_sqrt_nc, y * inv_sqrt_nc};
+    }
+  });
+  parallel_for(ngnc, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      I_eps_array[i] = {u01(draw(seed, kIeps, 2 * i)),
+                        -0.1 + 0.2 * u01(draw(seed, kIeps, 2 * i + 1))};
+      wtilde_array[i] = {0.5 + 2.5 * u01(draw(seed, kWtilde, 2 * i)),
+                         -0.05 + 0.1 * u01(draw(seed, kWtilde, 2 * i + 1))};
+    }
+  });
+  for (int ig = 0; ig < NC; ++ig)
+    vcoul[ig] = 1.0 / (0.01 + std::pow((double)ig / NC, 2.0 / 3.0));
+  for (int i = 0; i < NG; ++i) inv_igp_index[i] = i * nodes_per_group;
+
+  if (config == 2) {
+    for (int iw = 0; iw < 3; ++iw) wx_array[iw] = 4.0 + (iw - 1);
+    return GPP_OK;
+  }
+
+  // config 3: band-resolved energies, wx_array is [NB][3].
+  std::vector<double> en(NB);
+  for (int n = 0; n < NB; ++n) en[n] = -20.0 + 40.0 * u01(draw(seed, kEnergy, n));
+  std::sort(en.begin(), en.end());
+  for (int n = 0; n < NB; ++n)
+    for (int iw = 0; iw < 3; ++iw) {
+      double w = 4.0 - en[n] + (iw - 1);
+      if (std::fabs(w) < 1e-6) w = 1e-6;
+      wx_array[n * 3 + iw] = w;
+    }
+  if (NB > 0) {
+    parallel_for(ngnc, [&](size_t b, size_t e) {
+      for (size_t i = b; i < e; ++i) {
+        if (u01(draw(seed, kResPick, i)) >= 0.05) continue;
+        const uint64_t x = draw(seed, kResWhere, 2 * i);
+        const int n = (int)((x >> 11) % (uint64_t)NB);
+        const int iw = (int)((x >> 3) % 3);
+        const double off = -1e-4 + 2e-4 * u01(draw(seed, kResWhere, 2 * i + 1));
+        wtilde_array[i].re = wx_array[n * 3 + iw] + off;
+      }
+    });
+  }
+  return GPP_OK;
+}

@@ -1,0 +1,66 @@
+// gpp_internal.h — shared between the HIP kernels (gpp_device.hip) and the
+// host-side C ABI (gpp_api.cpp).  Nothing here is part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace gpp {
+
+// Physics constants of the reference (gppKer.cpp:182-188, 234).
+constexpr double kTo1 = 1e-6;                    // to1
+constexpr double kSexcut = 4.0;                  // sexcut
+constexpr double kLimitOne = 1.0 / (kTo1 * 4.0); // limitone = 2.5e5
+constexpr double kLimitTwo = 0.25;               // limittwo = 0.5^2
+constexpr double kOcc = 1.0;                     // occ
+
+// Semantics (mirrors GPP_SEM_* in include/gpp_sigma.h).
+enum Sem : int { kSemGppker = 0, kSemGppkerStrict = 1, kSemBgw = 2 };
+
+// Output slot layout of every partial-sum record (doubles):
+//   [0..5]  achtemp re/im for iw = 0,1,2
+//   [6..11] asxtemp re/im for iw = 0,1,2
+constexpr int kSlots = 12;
+
+// Device view of one sigma evaluation, after the prepass.
+struct DevProblem {
+  const double2 *aqsm;    // [NB][NC]
+  const double2 *aqsn;    // [NB][NC]
+  const double2 *ieps;    // [NG][NC]
+  const double2 *wtilde;  // [NG][NC]
+  const double *vcoul;    // [NC]
+  const int *igp_of;      // [NG]: resolved igp in [0, NC) or -1 (skipped)
+  const double *wx;       // [NB][3], clamped, band-resolved
+  const float2 *aqsn32;   // [NB][NC] FP32 copy (mixed precision only)
+  int NB, NV, NC, NG;
+};
+
+// Tile shape of the non-occupied kernel.
+struct TileCfg {
+  int k_ig = 16;          // G-vectors each lane keeps in registers
+  int waves = 4;          // waves per block (each takes its own k_ig run)
+};
+
+// Launchers (gpp_device.hip).  All enqueue on `s` and return hipError_t.
+hipError_t launch_prepass(const int *inv_igp_index, const int *indinv,
+                          const double *wx_in, int wx_per_band, int NB,
+                          int NC, int NG, int *igp_of, double *wx_out,
+                          unsigned long long *counter, hipStream_t s);
+hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, size_t n,
+                               hipStream_t s);
+int unocc_grid(const DevProblem &p, const TileCfg &t);
+hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
+                        int precision, int sem, const TileCfg &t,
+                        double res_thresh, double *partials,
+                        unsigned long long *fallback_count, hipStream_t s);
+int occ_grid(const DevProblem &p);
+hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
+                      double *partials, hipStream_t s);
+hipError_t launch_reduce(const double *partials, int nrec, double *out,
+                         hipStream_t s);
+int achstemp_records(const DevProblem &p, int sem);
+hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
+                           hipStream_t s);
+
+}  // namespace gpp

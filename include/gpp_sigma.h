@@ -1,0 +1,202 @@
+/*
+ * gpp_sigma.h — C ABI of the MI355X (gfx950) general-plasmon-pole (GPP)
+ * self-energy sigma sum.
+ *
+ * The library replaces the band loop of the gppKer mini-app
+ * (gppKer.cpp:270-361): for every band n1, G' column my_igp, frequency iw
+ * and G-vector ig it evaluates
+ *
+ *   wdiff = wx[n1][iw] - wtilde[my_igp][ig]
+ *   delw  = wtilde[my_igp][ig] * conj(wdiff) / |wdiff|^2
+ *   sch   = conj(aqsm[n1][igp]) * aqsn[n1][ig] * delw * I_eps[my_igp][ig]
+ *
+ * (plus the occupied-band screened-exchange branch for n1 < nvband,
+ * gppKer.cpp:16-62,122-146) and returns achtemp[3] and asxtemp[3].
+ *
+ * Every array argument uses the reference layout: row-major, ig fastest,
+ * complex numbers as interleaved {double re, im} (gppKer.cpp:215-221,
+ * 244-252).  gpp_c128 is layout-compatible with std::complex<double>, the
+ * reference's GPUComplex (Complex.h:18-22) and HIP's double2.
+ *
+ * Conventions (all entry points):
+ *   - return 0 (GPP_OK) on success, a negative GPP_E* code on failure;
+ *     nothing throws or exits across the ABI;
+ *   - the caller owns every input and output buffer;
+ *   - results are deterministic for a given shape, precision and tile
+ *     configuration (fixed-order two-pass reduction, no atomics);
+ *   - one gpp_handle per host thread; calls on one handle are serialised.
+ */
+#ifndef GPP_SIGMA_H
+#define GPP_SIGMA_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } gpp_c128;
+
+/* Return codes. */
+#define GPP_OK           0
+#define GPP_EINVAL      -1   /* bad shape, null pointer, bad option        */
+#define GPP_ENOMEM      -2   /* device allocation failed                   */
+#define GPP_EDEVICE     -3   /* HIP runtime error (launch, copy, device)   */
+#define GPP_ENONFINITE  -4   /* a result is NaN/Inf and the caller asked
+                                for GPP_FLAG_CHECK_FINITE                  */
+#define GPP_EUNSUPPORTED -5  /* option combination not implemented         */
+
+/* Arithmetic of the non-occupied (n1 >= nvband) sum. */
+#define GPP_PREC_FP64    0   /* complex128 throughout                      */
+#define GPP_PREC_MIXED   1   /* FP32 resolvent and products, FP64 band and
+                                cross-tile accumulation, FP64 for terms with
+                                |wx - wtilde|^2 below resonance_threshold  */
+
+/* Which variant of the reference physics to follow. */
+#define GPP_SEM_GPPKER         0 /* gppKer.cpp: unmasked non-occupied sch;
+                                    occupied branch of ssxt_scht_solver with
+                                    I_eps indexed [my_igp][ig];
+                                    asxtemp += ssx*occ*vcoul               */
+#define GPP_SEM_GPPKER_STRICT  1 /* as GPPKER but the occupied-branch sch and
+                                    cutoff read I_eps[my_igp*ngpown+ig], the
+                                    index gppKer.cpp:139 uses              */
+#define GPP_SEM_BGW            2 /* gppkernel.deslippe.09.30.2014.f90: masked
+                                    non-occupied sch, cden = wx^2 - wtilde^2,
+                                    cutoff for wx < 0, asxtemp -= ...      */
+
+/* Where the problem arrays live. */
+#define GPP_MEM_HOST     0   /* host pointers; the library stages them     */
+#define GPP_MEM_DEVICE   1   /* device pointers on the handle's device     */
+
+/* gpp_options.flags */
+#define GPP_FLAG_ALL_BANDS_UNOCC  1  /* treat every band as non-occupied
+                                        (achtemp_kernel semantics,
+                                        gppKer_gpuComplex.cpp:148-206)     */
+#define GPP_FLAG_SKIP_UNOCC       2  /* evaluate only n1 < nvband
+                                        (tillNvband_kernel semantics,
+                                        gppKer_gpuComplex.cpp:127-145)     */
+#define GPP_FLAG_CHECK_FINITE     4  /* return GPP_ENONFINITE on NaN/Inf   */
+#define GPP_FLAG_NO_ACHSTEMP      8  /* skip the static remainder achstemp */
+
+/* One sigma evaluation.  Shapes and arrays as in gppKer.cpp:157-264. */
+typedef struct {
+  int number_bands;               /* NB                                    */
+  int nvband;                     /* occupied bands, n1 < nvband           */
+  int ncouls;                     /* NC, G-vectors                         */
+  int ngpown;                     /* NG, G' columns owned (ncouls /
+                                     nodes_per_group, gppKer.cpp:163)      */
+  const gpp_c128 *aqsmtemp;       /* [NB][NC]                              */
+  const gpp_c128 *aqsntemp;       /* [NB][NC]                              */
+  const gpp_c128 *I_eps_array;    /* [NG][NC]                              */
+  const gpp_c128 *wtilde_array;   /* [NG][NC]                              */
+  const double   *vcoul;          /* [NC]                                  */
+  const int      *inv_igp_index;  /* [NG]  my_igp -> indigp               */
+  const int      *indinv;         /* [NC]  indigp -> igp; indigp == NC maps
+                                     to igp = NC-1 (gppKer.cpp:287-290)    */
+  const double   *wx_array;       /* [3] or [NB][3] (wx_per_band)         */
+  int wx_per_band;                /* 0: one wx[3] for all bands;
+                                     1: band-resolved wx[n1][3]            */
+} gpp_problem;
+
+typedef struct {
+  int precision;                  /* GPP_PREC_*                            */
+  int semantics;                  /* GPP_SEM_*                             */
+  int memory;                     /* GPP_MEM_*                             */
+  int flags;                      /* GPP_FLAG_*                            */
+  void *stream;                   /* hipStream_t for GPP_MEM_DEVICE work;
+                                     NULL = the handle's own stream        */
+  double resonance_threshold;     /* mixed: FP64 for |wdiff|^2 below this;
+                                     <= 0 selects the default 1e-2         */
+  int tile_ig;                    /* G-vectors per lane run (0 = tuned)    */
+  int waves_per_block;            /* 0 = tuned                             */
+} gpp_options;
+
+typedef struct {
+  gpp_c128 achtemp[3];
+  gpp_c128 asxtemp[3];
+  gpp_c128 achstemp;              /* static remainder (gppKer.cpp:64-120)  */
+  gpp_c128 *acht_n1_loc;          /* optional [NB] host buffer supplied by
+                                     the caller, NULL to skip
+                                     (gppKer.cpp:357,370-373)             */
+  double unocc_kernel_ms;         /* n1 >= nvband kernel, HIP events on the
+                                     stream it ran on                      */
+  double occ_kernel_ms;           /* n1 < nvband kernel                    */
+  double total_ms;                /* whole device section incl. reduction  */
+  double terms;                   /* NB * NG * NC * 3 evaluated            */
+  long long fp64_fallback_terms;  /* mixed: terms redone in FP64 (0 if not
+                                     counted)                              */
+} gpp_result;
+
+typedef struct gpp_handle_s *gpp_handle;
+
+/* Handle lifetime.  device < 0 selects the current HIP device. */
+int gpp_create(gpp_handle *h, int device);
+int gpp_destroy(gpp_handle h);
+const char *gpp_last_error(gpp_handle h);   /* never NULL */
+void gpp_default_options(gpp_options *o);
+
+/* The general entry point. */
+int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
+                 gpp_result *r);
+
+/* Many independent evaluations in one persistent launch (FP64 or mixed).
+ * All problems must live in the same memory space (o->memory). */
+int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
+                    const gpp_options *o, gpp_result *r);
+
+/*
+ * Drop-in for the band loop of gppKer.cpp:270-361.  Host pointers,
+ * FP64, GPP_SEM_GPPKER, a library-owned handle on the current device.
+ * wx_array holds 3 values shared by all bands, as gppKer.cpp:276-280
+ * computes them; |wx| < 1e-6 is clamped to 1e-6 as there.
+ * achtemp and asxtemp receive 3 values each.
+ */
+int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
+              const gpp_c128 *aqsmtemp, const gpp_c128 *aqsntemp,
+              const gpp_c128 *I_eps_array, const gpp_c128 *wtilde_array,
+              const double *vcoul, const int *inv_igp_index,
+              const int *indinv, const double *wx_array,
+              gpp_c128 *achtemp, gpp_c128 *asxtemp);
+
+/*
+ * Drop-ins for the reference's operator split (gppKer_gpuComplex.cpp).
+ * gpp_achtemp_kernel replaces achtemp_kernel (:148): every band taken as
+ * non-occupied, results added into the six out-references.
+ * gpp_tillNvband_kernel replaces tillNvband_kernel (:127): bands
+ * n1 < nvband, asxtemp[iw] += ssxt * occ * vcoul[igp].
+ * Both take host pointers and run FP64 on the current device.
+ */
+int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
+                       const gpp_c128 *wtilde_array, const gpp_c128 *aqsmtemp,
+                       const gpp_c128 *aqsntemp, const gpp_c128 *I_eps_array,
+                       const double *vcoul, const int *inv_igp_index,
+                       const int *indinv, int number_bands,
+                       double *achtemp_re0, double *achtemp_re1,
+                       double *achtemp_re2, double *achtemp_im0,
+                       double *achtemp_im1, double *achtemp_im2);
+
+int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
+                          gpp_c128 *asxtemp, const double *wx_array,
+                          const gpp_c128 *wtilde_array,
+                          const gpp_c128 *aqsmtemp, const gpp_c128 *aqsntemp,
+                          const gpp_c128 *I_eps_array, int ncouls, int nvband,
+                          int ngpown, const double *vcoul, int nstart,
+                          int nend, double occ, int number_bands);
+
+/* Synthetic inputs (host memory, caller-allocated, reference layout).
+ * config: 1 = gppKer.cpp constant fills; 2 = canonical seeded random;
+ * 3 = band-resolved energies with near-resonant wtilde (needs wx[NB][3]).
+ * nodes_per_group gives the inv_igp_index stride of configs 2 and 3.
+ * Deterministic for (config, seed, shape) on every host. */
+int gpp_fill_inputs(int config, unsigned long long seed, int number_bands,
+                    int ncouls, int ngpown, int nodes_per_group,
+                    gpp_c128 *aqsmtemp, gpp_c128 *aqsntemp,
+                    gpp_c128 *I_eps_array, gpp_c128 *wtilde_array,
+                    double *vcoul, int *inv_igp_index, int *indinv,
+                    double *wx_array);
+
+/* Version string of the library build. */
+const char *gpp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPP_SIGMA_H */

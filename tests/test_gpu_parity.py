@@ -1,0 +1,207 @@
+"""GPU parity tests: the HIP path, called through the C ABI, against the CPU
+oracle on the same inputs.  FP64 must agree within 1e-11 relative per
+achtemp/asxtemp entry; mixed precision is checked against its stated bound.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import gppk, gpp_oracle as orc, rel_err
+from test_oracle_cpu import closed_form
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-11
+
+
+def check(got, ref, tol=FP64_TOL, keys=("achtemp", "asxtemp")):
+    for k in keys:
+        if np.all(ref[k] == 0):
+            assert np.all(np.abs(got[k]) <= 1e-300), (k, got[k])
+            continue
+        e = rel_err(got[k], ref[k])
+        assert e <= tol, (k, e, got[k], ref[k])
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("shape", [(64, 2, 2048, 8), (8, 2, 64, 4), (5, 5, 40, 8),
+                                   (12, 0, 100, 7), (3, 7, 30, 3), (9, 3, 1000, 13)])
+@pytest.mark.parametrize("sem", ["gppker", "strict", "bgw"])
+def test_constfill_closed_form(handle, shape, sem):
+    nb, nv, nc, npg = shape
+    say("constfill", shape, sem)
+    inp = gppk.fill_inputs(1, nb, nc, npg, nvband=nv)
+    got = handle.sigma(inp, semantics=sem)
+    ach, asx = closed_form(nb, nv, nc, npg, "bgw" if sem == "bgw" else "gppker")
+    assert rel_err(got["achtemp"], ach) < 1e-12
+    if min(nv, nb) > 0:
+        assert rel_err(got["asxtemp"], asx) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(6, 2, 256, 4), (5, 1, 1000, 10), (4, 0, 777, 7),
+                                   (3, 3, 130, 2), (2, 5, 300, 3), (7, 2, 4096, 32)])
+@pytest.mark.parametrize("sem", ["gppker", "strict", "bgw"])
+def test_random_fp64_vs_oracle(handle, shape, sem):
+    nb, nv, nc, npg = shape
+    say("random", shape, sem)
+    inp = gppk.fill_inputs(2, nb, nc, npg, nvband=nv, seed=nb * 31 + nc)
+    got = handle.sigma(inp, semantics=sem)
+    ref = orc.sigma(inp, sem)
+    check(got, ref)
+    assert abs(got["achstemp"] - ref["achstemp"]) <= 1e-11 * abs(ref["achstemp"])
+
+
+@pytest.mark.parametrize("tile", [(4, 1), (8, 2), (8, 4), (16, 4), (32, 2), (16, 16)])
+def test_tile_shapes_agree(handle, tile):
+    k, w = tile
+    say("tile", tile)
+    inp = gppk.fill_inputs(2, 5, 1500, 10, nvband=1, seed=9)
+    ref = orc.sigma(inp)
+    got = handle.sigma(inp, tile_ig=k, waves_per_block=w)
+    check(got, ref)
+
+
+def test_band_resolved_fp64(handle):
+    say("band-resolved")
+    inp = gppk.fill_inputs(3, 24, 600, 6, nvband=4, seed=4)
+    for sem in ("gppker", "bgw"):
+        check(handle.sigma(inp, semantics=sem), orc.sigma(inp, sem))
+
+
+def test_deterministic(handle):
+    say("determinism")
+    inp = gppk.fill_inputs(2, 8, 2048, 16, nvband=2, seed=2)
+    a = handle.sigma(inp)
+    b = handle.sigma(inp)
+    assert np.array_equal(a["achtemp"], b["achtemp"])
+    assert np.array_equal(a["asxtemp"], b["asxtemp"])
+
+
+def test_skipped_rows_and_index_edge(handle):
+    say("index edges")
+    inp = gppk.fill_inputs(2, 3, 320, 4, nvband=1, seed=3)
+    inv = inp["inv_igp_index"].copy()
+    inv[1] = -5
+    inv[2] = 320
+    inv[3] = 10_000
+    q = dict(inp, inv_igp_index=inv)
+    check(handle.sigma(q), orc.sigma(q))
+
+
+def test_exact_resonance_nan_pattern(handle):
+    say("resonance")
+    inp = gppk.fill_inputs(2, 3, 256, 4, nvband=0, seed=5)
+    w = inp["wtilde_array"].copy()
+    w[0, 5] = 4.0 + 0.0j
+    q = dict(inp, wtilde_array=w)
+    got = handle.sigma(q)
+    ref = orc.sigma(q)
+    assert np.isnan(got["achtemp"][1]) and np.isnan(ref["achtemp"][1])
+    assert rel_err(got["achtemp"][[0, 2]], ref["achtemp"][[0, 2]]) <= FP64_TOL
+    g2 = handle.sigma(q, semantics="bgw")
+    check(g2, orc.sigma(q, "bgw"))
+
+
+def test_check_finite_flag(handle):
+    inp = gppk.fill_inputs(2, 2, 256, 4, nvband=0, seed=5)
+    w = inp["wtilde_array"].copy()
+    w[0, 5] = 4.0 + 0.0j
+    with pytest.raises(gppk.GppError) as ei:
+        handle.sigma(dict(inp, wtilde_array=w), flags=gppk.FLAG_CHECK_FINITE)
+    assert ei.value.code == gppk.GPP_ENONFINITE
+
+
+def test_invalid_arguments_are_errors(handle):
+    inp = gppk.fill_inputs(2, 2, 256, 4, nvband=0, seed=5)
+    with pytest.raises(gppk.GppError) as ei:
+        handle.sigma(inp, tile_ig=5)
+    assert ei.value.code == gppk.GPP_EINVAL
+    with pytest.raises(gppk.GppError) as ei:
+        handle.sigma(inp, precision="mixed", semantics="bgw")
+    assert ei.value.code == gppk.GPP_EUNSUPPORTED
+
+
+def test_mixed_precision_error(handle):
+    say("mixed")
+    for cfg in (2, 3):
+        inp = gppk.fill_inputs(cfg, 16, 2048, 8, nvband=2, seed=8)
+        ref = orc.sigma(inp)
+        got = handle.sigma(inp, precision="mixed")
+        e = rel_err(got["achtemp"], ref["achtemp"])
+        say(f"config {cfg} mixed achtemp rel err {e:.3e} fallback terms "
+            f"{got['fp64_fallback_terms']}")
+        assert e < 2e-6
+        # occupied bands always run in FP64
+        assert rel_err(got["asxtemp"], ref["asxtemp"]) <= FP64_TOL
+        if cfg == 3:
+            assert got["fp64_fallback_terms"] > 0
+
+
+def test_dropin_entry_points(handle):
+    say("drop-in")
+    lib = gppk.load_library()
+    inp = gppk.fill_inputs(2, 5, 512, 8, nvband=2, seed=12)
+    p = {k: np.ascontiguousarray(v) for k, v in inp.items() if isinstance(v, np.ndarray)}
+    nb, nc, ng = inp["number_bands"], inp["ncouls"], inp["ngpown"]
+    ach = (gppk.c128 * 3)()
+    asx = (gppk.c128 * 3)()
+    P = lambda k: p[k].ctypes.data
+    rc = lib.gpp_sigma(nb, 2, nc, ng, P("aqsmtemp"), P("aqsntemp"), P("I_eps_array"),
+                       P("wtilde_array"), P("vcoul"), P("inv_igp_index"),
+                       P("indinv"), P("wx_array"), ctypes.byref(ach), ctypes.byref(asx))
+    assert rc == 0
+    ref = orc.sigma(inp)
+    check({"achtemp": np.array([complex(z.re, z.im) for z in ach]),
+           "asxtemp": np.array([complex(z.re, z.im) for z in asx])}, ref)
+    # achtemp_kernel: all bands unoccupied, adds into the six references
+    d = [ctypes.c_double(1.0) for _ in range(6)]
+    rc = lib.gpp_achtemp_kernel(nc, ng, P("wx_array"), P("wtilde_array"), P("aqsmtemp"),
+                                P("aqsntemp"), P("I_eps_array"), P("vcoul"),
+                                P("inv_igp_index"), P("indinv"), nb,
+                                *[ctypes.byref(x) for x in d])
+    assert rc == 0
+    r2 = orc.sigma(inp, flags=orc.ALL_BANDS_UNOCC)["achtemp"]
+    got = np.array([d[0].value + 1j * d[3].value, d[1].value + 1j * d[4].value,
+                    d[2].value + 1j * d[5].value]) - (1 + 1j)
+    assert rel_err(got, r2) <= FP64_TOL
+    # tillNvband_kernel: occupied bands only, asxtemp += ssxt*occ*vcoul
+    asx2 = (gppk.c128 * 3)()
+    rc = lib.gpp_tillNvband_kernel(P("inv_igp_index"), P("indinv"), ctypes.byref(asx2),
+                                   P("wx_array"), P("wtilde_array"), P("aqsmtemp"),
+                                   P("aqsntemp"), P("I_eps_array"), nc, 2, ng,
+                                   P("vcoul"), 0, 3, 1.0, nb)
+    assert rc == 0
+    r3 = orc.sigma(inp, "strict", flags=orc.SKIP_UNOCC)["asxtemp"]
+    assert rel_err(np.array([complex(z.re, z.im) for z in asx2]), r3) <= FP64_TOL
+
+
+def test_device_memory_path(handle):
+    import torch
+    say("device memory")
+    inp = gppk.fill_inputs(2, 6, 1024, 8, nvband=2, seed=13)
+    dev = {k: (torch.from_numpy(np.ascontiguousarray(v)).cuda() if isinstance(v, np.ndarray) else v)
+           for k, v in inp.items()}
+    s = torch.cuda.current_stream()
+    got = handle.sigma(dev, memory=gppk.MEM_DEVICE, stream=s.cuda_stream)
+    check(got, orc.sigma(inp))
+
+
+def test_canonical_shape_band_subset(handle):
+    """512/2/32768/20 inputs; the oracle runs the first 3 bands (2 occupied,
+    1 unoccupied) at full ncouls and ngpown; the GPU runs the same slice and
+    the full problem, whose sum must equal the sum of band slices."""
+    say("canonical subset")
+    inp = gppk.fill_inputs(2, 512, 32768, 20, nvband=2, seed=2024)
+    sub = gppk.band_slice(inp, 0, 3)
+    got = handle.sigma(sub)
+    ref = orc.sigma(sub, want_achstemp=False)
+    check(got, ref)
+    full = handle.sigma(inp)
+    parts = [handle.sigma(gppk.band_slice(inp, a, b)) for a, b in ((0, 100), (100, 512))]
+    for k in ("achtemp", "asxtemp"):
+        assert rel_err(parts[0][k] + parts[1][k], full[k]) <= 1e-12
