@@ -40,6 +40,7 @@ FLAG_ALL_BANDS_UNOCC = 1
 FLAG_SKIP_UNOCC = 2
 FLAG_CHECK_FINITE = 4
 FLAG_NO_ACHSTEMP = 8
+FLAG_AUTOTUNE = 16
 
 SEMANTICS = {"gppker": SEM_GPPKER, "strict": SEM_GPPKER_STRICT, "bgw": SEM_BGW}
 PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
@@ -47,8 +48,9 @@ PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
 # Exported symbols, one per declaration of include/gpp_sigma.h.
 EXPORTS = (
     "gpp_create", "gpp_destroy", "gpp_last_error", "gpp_default_options",
-    "gpp_sigma_ex", "gpp_sigma_batch", "gpp_sigma", "gpp_achtemp_kernel",
-    "gpp_tillNvband_kernel", "gpp_fill_inputs", "gpp_version",
+    "gpp_sigma_ex", "gpp_sigma_batch", "gpp_autotune", "gpp_sigma",
+    "gpp_achtemp_kernel", "gpp_tillNvband_kernel", "gpp_fill_inputs",
+    "gpp_version",
 )
 
 
@@ -110,6 +112,8 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
                                  C.POINTER(Result)]
     lib.gpp_sigma_batch.argtypes = [vp, C.POINTER(Problem), ip,
                                     C.POINTER(Options), C.POINTER(Result)]
+    lib.gpp_autotune.argtypes = [vp, C.POINTER(Problem), C.POINTER(Options),
+                                 C.POINTER(ip), C.POINTER(ip)]
     lib.gpp_sigma.argtypes = [ip, ip, ip, ip] + [vp] * 10
     lib.gpp_achtemp_kernel.argtypes = [ip, ip] + [vp] * 8 + [ip] + [vp] * 6
     lib.gpp_tillNvband_kernel.argtypes = ([vp] * 8 + [ip, ip, ip, vp, ip, ip,
@@ -196,10 +200,8 @@ class Handle:
     def last_error(self) -> str:
         return self.lib.gpp_last_error(self.h).decode()
 
-    def sigma(self, inp: dict, precision="fp64", semantics="gppker",
-              memory=None, flags=0, stream=None, resonance_threshold=0.0,
-              tile_ig=0, waves_per_block=0) -> dict:
-        prob = make_problem(inp)
+    def _options(self, inp, precision, semantics, memory, flags, stream,
+                 resonance_threshold, tile_ig, waves_per_block) -> Options:
         if memory is None:
             memory = MEM_DEVICE if hasattr(inp["aqsmtemp"], "data_ptr") else MEM_HOST
         opt = Options()
@@ -212,10 +214,10 @@ class Handle:
         opt.resonance_threshold = resonance_threshold
         opt.tile_ig = tile_ig
         opt.waves_per_block = waves_per_block
-        res = Result()
-        rc = self.lib.gpp_sigma_ex(self.h, C.byref(prob), C.byref(opt), C.byref(res))
-        if rc != GPP_OK:
-            raise GppError(rc, self.last_error())
+        return opt
+
+    @staticmethod
+    def _result(res: Result) -> dict:
         return {
             "achtemp": np.array([_cplx(z) for z in res.achtemp]),
             "asxtemp": np.array([_cplx(z) for z in res.asxtemp]),
@@ -226,6 +228,54 @@ class Handle:
             "terms": res.terms,
             "fp64_fallback_terms": res.fp64_fallback_terms,
         }
+
+    def sigma(self, inp: dict, precision="fp64", semantics="gppker",
+              memory=None, flags=0, stream=None, resonance_threshold=0.0,
+              tile_ig=0, waves_per_block=0, acht_n1_loc=False) -> dict:
+        """One evaluation.  With acht_n1_loc=True the per-band log
+        (gppKer.cpp:357) is returned too."""
+        prob = make_problem(inp)
+        opt = self._options(inp, precision, semantics, memory, flags, stream,
+                            resonance_threshold, tile_ig, waves_per_block)
+        res = Result()
+        n1 = None
+        if acht_n1_loc:
+            n1 = np.zeros(max(prob.number_bands, 1), np.complex128)
+            res.acht_n1_loc = n1.ctypes.data
+        rc = self.lib.gpp_sigma_ex(self.h, C.byref(prob), C.byref(opt), C.byref(res))
+        if rc != GPP_OK:
+            raise GppError(rc, self.last_error())
+        out = self._result(res)
+        if n1 is not None:
+            out["acht_n1_loc"] = n1[: prob.number_bands]
+        return out
+
+    def batch(self, inputs: list, precision="fp64", semantics="gppker",
+              memory=None, flags=0, stream=None, resonance_threshold=0.0,
+              tile_ig=0) -> list:
+        """Many independent evaluations in one persistent launch."""
+        probs = (Problem * len(inputs))(*[make_problem(d) for d in inputs])
+        opt = self._options(inputs[0], precision, semantics, memory, flags,
+                            stream, resonance_threshold, tile_ig, 0)
+        res = (Result * len(inputs))()
+        rc = self.lib.gpp_sigma_batch(self.h, probs, len(inputs), C.byref(opt), res)
+        if rc != GPP_OK:
+            raise GppError(rc, self.last_error())
+        return [self._result(r) for r in res]
+
+    def autotune(self, inp: dict, precision="fp64", semantics="gppker",
+                 memory=None, stream=None) -> tuple:
+        """Run the tile autotuner for this problem's shape class; returns
+        (tile_ig, waves_per_block), also cached in the handle."""
+        prob = make_problem(inp)
+        opt = self._options(inp, precision, semantics, memory, 0, stream,
+                            0.0, 0, 0)
+        k, w = C.c_int(), C.c_int()
+        rc = self.lib.gpp_autotune(self.h, C.byref(prob), C.byref(opt),
+                                   C.byref(k), C.byref(w))
+        if rc != GPP_OK:
+            raise GppError(rc, self.last_error())
+        return k.value, w.value
 
 
 def fill_inputs(config: int, number_bands: int, ncouls: int,

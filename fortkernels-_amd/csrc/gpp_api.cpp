@@ -1,5 +1,7 @@
 // gpp_api.cpp — the C ABI of include/gpp_sigma.h: handle lifetime, input
-// validation, host->device staging, kernel sequencing, result read-back.
+// validation, host->device staging, kernel sequencing, result read-back,
+// the batched persistent launch (captured in a hipGraph) and the tile
+// autotuner.
 //
 // The call sequence per evaluation replaces the reference's band loop
 // (gppKer.cpp:270-361):
@@ -8,20 +10,26 @@
 //   -> occupied kernel over bands [0, nvband)           (gppKer.cpp:301-312)
 //   -> static remainder achstemp                        (gppKer.cpp:274)
 //   -> fixed-order reduction of all block records       (gppKer.cpp:365-373)
-// Device memory is owned by the handle and grows on demand; it is never
-// allocated or freed inside the timed kernel sequence.
+// Device memory is owned by the handle and grows on demand; nothing is
+// allocated or freed between the first and the last kernel of a call.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
+#include <vector>
 
 #include "../../include/gpp_sigma.h"
 #include "gpp_internal.h"
 
+using gpp::BatchItem;
 using gpp::DevProblem;
 using gpp::TileCfg;
 
@@ -32,16 +40,33 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// Tuned tile per shape class: (precision, masked, log2 NC, log2 NG).
+using TuneKey = std::tuple<int, int, int, int>;
+
+// Mixed precision: |wx - wtilde|^2 below which a term is redone in FP64.
+// The FP32 path forms wx - wtilde in FP64 before rounding, so its relative
+// error does not grow near resonance; the threshold only has to catch
+// FP32 underflow, exact resonances and NaN.
+constexpr double kDefaultResonance = 1e-6;
+
 }  // namespace
 
 struct gpp_handle_s {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {};
-  DevBuf in[8];       // staging of host inputs
-  DevBuf igp_of, wx, part, out, aqsn32, count;
-  double *host_out = nullptr;        // pinned, 2*kSlots doubles
-  unsigned long long *host_count = nullptr;
+  hipEvent_t ev[6] = {};
+  DevBuf stage;                 // host-input staging arena
+  DevBuf work;                  // per-call workspace arena
+  DevBuf part;                  // block records
+  DevBuf n1log;                 // per-band records (acht_n1_loc)
+  DevBuf meta;                  // batch: problem table, items, record ranges
+  double *host_out = nullptr;   // pinned result slots
+  size_t host_out_cap = 0;      // doubles
+  std::map<TuneKey, TileCfg> tuned;
+  std::string tune_file;
+  // Cached graph of the last batch (replayed when the call is identical).
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_sig;
   std::string err;
 };
 
@@ -70,9 +95,11 @@ int set_err(gpp_handle h, int code, const char *fmt, ...) {
   } while (0)
 
 int ensure(gpp_handle h, DevBuf &b, size_t bytes) {
-  if (bytes == 0) bytes = 16;
+  if (bytes == 0) bytes = 256;
   if (b.cap >= bytes) return GPP_OK;
   if (b.ptr) {
+    HIP_OR_RET(h, hipStreamSynchronize(h->stream));
+    HIP_OR_RET(h, hipDeviceSynchronize());
     HIP_OR_RET(h, hipFree(b.ptr));
     b.ptr = nullptr;
     b.cap = 0;
@@ -84,6 +111,27 @@ int ensure(gpp_handle h, DevBuf &b, size_t bytes) {
   b.cap = want;
   return GPP_OK;
 }
+
+int ensure_host_out(gpp_handle h, size_t doubles) {
+  if (h->host_out_cap >= doubles) return GPP_OK;
+  if (h->host_out) (void)hipHostFree(h->host_out);
+  h->host_out = nullptr;
+  h->host_out_cap = 0;
+  HIP_OR_RET(h, hipHostMalloc((void **)&h->host_out, doubles * sizeof(double),
+                              hipHostMallocDefault));
+  h->host_out_cap = doubles;
+  return GPP_OK;
+}
+
+// Bump allocator over one DevBuf, 256-byte aligned slices.
+struct Arena {
+  size_t used = 0;
+  size_t take(size_t bytes) {
+    const size_t at = used;
+    used += (bytes + 255) / 256 * 256;
+    return at;
+  }
+};
 
 int validate(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
   if (!p || !o) return set_err(h, GPP_EINVAL, "null problem or options");
@@ -119,26 +167,201 @@ int validate(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
   if (o->tile_ig != 0 && o->tile_ig != 4 && o->tile_ig != 8 &&
       o->tile_ig != 16 && o->tile_ig != 32)
     return set_err(h, GPP_EINVAL, "tile_ig must be 0, 4, 8, 16 or 32");
-  if (o->waves_per_block < 0 || o->waves_per_block > 16)
-    return set_err(h, GPP_EINVAL, "waves_per_block must be in [0,16]");
+  // The kernels are compiled for at most 256 threads per block.
+  if (o->waves_per_block < 0 || o->waves_per_block > 4)
+    return set_err(h, GPP_EINVAL, "waves_per_block must be in [0,4]");
   return GPP_OK;
 }
 
-TileCfg pick_tile(const gpp_problem *p, const gpp_options *o) {
+int ilog2_ceil(long long x) {
+  int l = 0;
+  while ((1LL << l) < x) ++l;
+  return l;
+}
+
+TuneKey tune_key(const gpp_problem *p, const gpp_options *o) {
+  return TuneKey(o->precision, o->semantics == GPP_SEM_BGW ? 1 : 0,
+                 ilog2_ceil(p->ncouls), ilog2_ceil(p->ngpown > 0 ? p->ngpown : 1));
+}
+
+TileCfg default_tile(int precision) {
   TileCfg t;
-  t.k_ig = o->precision == GPP_PREC_MIXED ? 16 : 8;
+  t.k_ig = precision == GPP_PREC_MIXED ? 16 : 8;
   t.waves = 4;
+  return t;
+}
+
+TileCfg pick_tile(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
+  TileCfg t = default_tile(o->precision);
+  auto it = h->tuned.find(tune_key(p, o));
+  if (it != h->tuned.end()) t = it->second;
   if (o->tile_ig) t.k_ig = o->tile_ig;
   if (o->waves_per_block) t.waves = o->waves_per_block;
-  (void)p;
   return t;
+}
+
+void load_tune_file(gpp_handle h) {
+  const char *f = std::getenv("GPP_TUNE_CACHE");
+  if (!f || !*f) return;
+  h->tune_file = f;
+  FILE *fp = std::fopen(f, "r");
+  if (!fp) return;
+  int a, b, c, d, k, w;
+  while (std::fscanf(fp, "%d %d %d %d %d %d", &a, &b, &c, &d, &k, &w) == 6) {
+    TileCfg t;
+    t.k_ig = k;
+    t.waves = w;
+    h->tuned[TuneKey(a, b, c, d)] = t;
+  }
+  std::fclose(fp);
+}
+
+void save_tune_entry(gpp_handle h, const TuneKey &key, const TileCfg &t) {
+  if (h->tune_file.empty()) return;
+  FILE *fp = std::fopen(h->tune_file.c_str(), "a");
+  if (!fp) return;
+  std::fprintf(fp, "%d %d %d %d %d %d\n", std::get<0>(key), std::get<1>(key),
+               std::get<2>(key), std::get<3>(key), t.k_ig, t.waves);
+  std::fclose(fp);
+}
+
+// Byte sizes of the eight problem arrays, in gpp_problem order.
+void array_bytes(const gpp_problem *p, size_t bytes[8]) {
+  const size_t nbnc = (size_t)p->number_bands * p->ncouls;
+  const size_t ngnc = (size_t)p->ngpown * p->ncouls;
+  bytes[0] = nbnc * 16;
+  bytes[1] = nbnc * 16;
+  bytes[2] = ngnc * 16;
+  bytes[3] = ngnc * 16;
+  bytes[4] = (size_t)p->ncouls * 8;
+  bytes[5] = (size_t)p->ngpown * 4;
+  bytes[6] = (size_t)p->ncouls * 4;
+  bytes[7] = (size_t)(p->wx_per_band ? p->number_bands * 3 : 3) * 8;
+}
+
+void array_ptrs(const gpp_problem *p, const void *src[8]) {
+  src[0] = p->aqsmtemp;
+  src[1] = p->aqsntemp;
+  src[2] = p->I_eps_array;
+  src[3] = p->wtilde_array;
+  src[4] = p->vcoul;
+  src[5] = p->inv_igp_index;
+  src[6] = p->indinv;
+  src[7] = p->wx_array;
+}
+
+// Everything the kernels need for one problem, in device memory.
+struct Prepared {
+  DevProblem dp;
+  const int *inv = nullptr;
+  const int *indinv = nullptr;
+  const double *wx_in = nullptr;
+  int wx_per_band = 0;
+};
+
+// Plans the device layout of `n` problems: staging (host inputs) in
+// h->stage, igp/wx/aqsn32 tables in h->work.  Enqueues the H2D copies.
+int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
+            bool mixed, hipStream_t s, std::vector<Prepared> &out) {
+  int rc;
+  out.assign(n, Prepared());
+  // Staging of host inputs.
+  Arena sa;
+  std::vector<size_t> soff((size_t)n * 8);
+  if (o->memory == GPP_MEM_HOST) {
+    for (int i = 0; i < n; ++i) {
+      size_t bytes[8];
+      array_bytes(p + i, bytes);
+      for (int j = 0; j < 8; ++j) soff[(size_t)i * 8 + j] = sa.take(bytes[j]);
+    }
+    if ((rc = ensure(h, h->stage, sa.used))) return rc;
+  }
+  Arena wa;
+  std::vector<size_t> woff((size_t)n * 3);
+  for (int i = 0; i < n; ++i) {
+    const size_t nbnc = (size_t)p[i].number_bands * p[i].ncouls;
+    woff[(size_t)i * 3 + 0] = wa.take((size_t)p[i].ngpown * sizeof(int));
+    woff[(size_t)i * 3 + 1] = wa.take((size_t)p[i].number_bands * 3 * 8);
+    woff[(size_t)i * 3 + 2] = mixed ? wa.take(nbnc * sizeof(float2)) : 0;
+  }
+  if ((rc = ensure(h, h->work, wa.used))) return rc;
+  char *sbase = (char *)h->stage.ptr;
+  char *wbase = (char *)h->work.ptr;
+  for (int i = 0; i < n; ++i) {
+    const void *src[8];
+    array_ptrs(p + i, src);
+    const void *dev[8];
+    if (o->memory == GPP_MEM_HOST) {
+      size_t bytes[8];
+      array_bytes(p + i, bytes);
+      for (int j = 0; j < 8; ++j) {
+        void *d = sbase + soff[(size_t)i * 8 + j];
+        if (bytes[j] && src[j])
+          HIP_OR_RET(h, hipMemcpyAsync(d, src[j], bytes[j],
+                                       hipMemcpyHostToDevice, s));
+        dev[j] = d;
+      }
+    } else {
+      for (int j = 0; j < 8; ++j) dev[j] = src[j];
+    }
+    Prepared &q = out[i];
+    q.dp.aqsm = (const double2 *)dev[0];
+    q.dp.aqsn = (const double2 *)dev[1];
+    q.dp.ieps = (const double2 *)dev[2];
+    q.dp.wtilde = (const double2 *)dev[3];
+    q.dp.vcoul = (const double *)dev[4];
+    q.inv = (const int *)dev[5];
+    q.indinv = (const int *)dev[6];
+    q.wx_in = (const double *)dev[7];
+    q.wx_per_band = p[i].wx_per_band;
+    q.dp.igp_of = (const int *)(wbase + woff[(size_t)i * 3 + 0]);
+    q.dp.wx = (const double *)(wbase + woff[(size_t)i * 3 + 1]);
+    q.dp.aqsn32 =
+        mixed ? (const float2 *)(wbase + woff[(size_t)i * 3 + 2]) : nullptr;
+    q.dp.NB = p[i].number_bands;
+    q.dp.NV = std::min(p[i].nvband, p[i].number_bands);
+    q.dp.NC = p[i].ncouls;
+    q.dp.NG = p[i].ngpown;
+  }
+  return GPP_OK;
+}
+
+// The fallback counter lives in the first 64 bytes of h->part (the batch
+// work counter in the next 64).
+unsigned long long *counter_ptr(gpp_handle h) {
+  return (unsigned long long *)h->part.ptr;
+}
+
+// Enqueue prepass (+ FP32 copy) for one prepared problem.
+int enqueue_prepass(gpp_handle h, const Prepared &q, bool mixed,
+                    unsigned long long *counter, hipStream_t s) {
+  HIP_OR_RET(h, gpp::launch_prepass(q.inv, q.indinv, q.wx_in, q.wx_per_band,
+                                    q.dp.NB, q.dp.NC, q.dp.NG,
+                                    (int *)q.dp.igp_of, (double *)q.dp.wx,
+                                    counter, s));
+  if (mixed && q.dp.NB > q.dp.NV)
+    HIP_OR_RET(h, gpp::launch_aqsn_to_fp32(
+                      q.dp.aqsn, (float2 *)q.dp.aqsn32,
+                      (size_t)q.dp.NB * q.dp.NC, s));
+  return GPP_OK;
+}
+
+void fill_result(const double *ho, const double *hach, gpp_result *r) {
+  for (int iw = 0; iw < 3; ++iw) {
+    r->achtemp[iw].re = ho[2 * iw];
+    r->achtemp[iw].im = ho[2 * iw + 1];
+    r->asxtemp[iw].re = ho[6 + 2 * iw];
+    r->asxtemp[iw].im = ho[6 + 2 * iw + 1];
+  }
+  r->achstemp.re = hach[0];
+  r->achstemp.im = hach[1];
 }
 
 }  // namespace
 
 extern "C" {
 
-const char *gpp_version(void) { return "gpp-mi355x 0.1 (gfx950)"; }
+const char *gpp_version(void) { return "gpp-mi355x 0.2 (gfx950)"; }
 
 void gpp_default_options(gpp_options *o) {
   if (!o) return;
@@ -179,32 +402,27 @@ int gpp_create(gpp_handle *out, int device) {
       return GPP_EDEVICE;
     }
   }
-  if (hipHostMalloc((void **)&h->host_out, 2 * gpp::kSlots * sizeof(double),
-                    hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void **)&h->host_count, sizeof(unsigned long long),
-                    hipHostMallocDefault) != hipSuccess) {
+  if (ensure_host_out(h, 4 * gpp::kSlots) != GPP_OK) {
     gpp_destroy(h);
     return GPP_ENOMEM;
   }
+  load_tune_file(h);
   *out = h;
   return GPP_OK;
 }
 
 int gpp_destroy(gpp_handle h) {
   if (!h) return GPP_OK;
-  hipSetDevice(h->device);
-  if (h->stream) hipStreamSynchronize(h->stream);
-  for (auto &b : h->in)
-    if (b.ptr) hipFree(b.ptr);
-  DevBuf *bufs[] = {&h->igp_of, &h->wx, &h->part, &h->out, &h->aqsn32,
-                    &h->count};
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  DevBuf *bufs[] = {&h->stage, &h->work, &h->part, &h->n1log, &h->meta};
   for (DevBuf *b : bufs)
-    if (b->ptr) hipFree(b->ptr);
+    if (b->ptr) (void)hipFree(b->ptr);
   for (auto &ev : h->ev)
-    if (ev) hipEventDestroy(ev);
-  if (h->host_out) hipHostFree(h->host_out);
-  if (h->host_count) hipHostFree(h->host_count);
-  if (h->stream) hipStreamDestroy(h->stream);
+    if (ev) (void)hipEventDestroy(ev);
+  if (h->host_out) (void)hipHostFree(h->host_out);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return GPP_OK;
 }
@@ -222,105 +440,69 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   int rc = validate(h, p, o);
   if (rc) return rc;
   HIP_OR_RET(h, hipSetDevice(h->device));
-
-  const int NB = p->number_bands, NC = p->ncouls, NG = p->ngpown;
-  const int NV = p->nvband < NB ? p->nvband : NB;
-  const size_t nbnc = (size_t)NB * NC, ngnc = (size_t)NG * NC;
   hipStream_t s = (o->memory == GPP_MEM_DEVICE && o->stream)
                       ? (hipStream_t)o->stream
                       : h->stream;
-
-  // Device views of the inputs.
-  const void *src[8] = {p->aqsmtemp, p->aqsntemp,      p->I_eps_array,
-                        p->wtilde_array, p->vcoul,     p->inv_igp_index,
-                        p->indinv,     p->wx_array};
-  const size_t bytes[8] = {nbnc * 16,
-                           nbnc * 16,
-                           ngnc * 16,
-                           ngnc * 16,
-                           (size_t)NC * 8,
-                           (size_t)NG * 4,
-                           (size_t)NC * 4,
-                           (size_t)(p->wx_per_band ? NB * 3 : 3) * 8};
-  const void *dev[8];
-  if (o->memory == GPP_MEM_HOST) {
-    for (int i = 0; i < 8; ++i) {
-      if ((rc = ensure(h, h->in[i], bytes[i]))) return rc;
-      if (bytes[i] && src[i])
-        HIP_OR_RET(h, hipMemcpyAsync(h->in[i].ptr, src[i], bytes[i],
-                                     hipMemcpyHostToDevice, s));
-      dev[i] = h->in[i].ptr;
-    }
-  } else {
-    for (int i = 0; i < 8; ++i) dev[i] = src[i];
-  }
-
-  if ((rc = ensure(h, h->igp_of, (size_t)NG * sizeof(int)))) return rc;
-  if ((rc = ensure(h, h->wx, (size_t)NB * 3 * sizeof(double)))) return rc;
-  if ((rc = ensure(h, h->out, 2 * gpp::kSlots * sizeof(double)))) return rc;
-  if ((rc = ensure(h, h->count, sizeof(unsigned long long)))) return rc;
-
-  DevProblem dp;
-  dp.aqsm = (const double2 *)dev[0];
-  dp.aqsn = (const double2 *)dev[1];
-  dp.ieps = (const double2 *)dev[2];
-  dp.wtilde = (const double2 *)dev[3];
-  dp.vcoul = (const double *)dev[4];
-  dp.igp_of = (const int *)h->igp_of.ptr;
-  dp.wx = (const double *)h->wx.ptr;
-  dp.aqsn32 = nullptr;
-  dp.NB = NB;
-  dp.NV = NV;
-  dp.NC = NC;
-  dp.NG = NG;
-
-  const TileCfg tile = pick_tile(p, o);
+  const bool mixed = o->precision == GPP_PREC_MIXED;
   const bool all_unocc = o->flags & GPP_FLAG_ALL_BANDS_UNOCC;
   const bool skip_unocc = o->flags & GPP_FLAG_SKIP_UNOCC;
   const bool want_ach = !(o->flags & GPP_FLAG_NO_ACHSTEMP);
-  const int n1_unocc_begin = all_unocc ? 0 : NV;
-  const int n1_occ_end = all_unocc ? 0 : NV;
-  const bool run_unocc = !skip_unocc && n1_unocc_begin < NB;
-  const bool run_occ = n1_occ_end > 0;
+  const bool want_n1 = r->acht_n1_loc != nullptr;
+
+  if ((o->flags & GPP_FLAG_AUTOTUNE) && !o->tile_ig && !o->waves_per_block &&
+      !h->tuned.count(tune_key(p, o))) {
+    int k, w;
+    if ((rc = gpp_autotune(h, p, o, &k, &w))) return rc;
+  }
+  const TileCfg tile = pick_tile(h, p, o);
+
+  std::vector<Prepared> prep;
+  if ((rc = prepare(h, p, 1, o, mixed, s, prep))) return rc;
+  const DevProblem &dp = prep[0].dp;
+  const int NB = dp.NB, NV = dp.NV;
+
+  const int n1u_begin = all_unocc ? 0 : NV;
+  const int n1o_end = all_unocc ? 0 : NV;
+  const bool run_unocc = !skip_unocc && n1u_begin < NB;
+  const bool run_occ = n1o_end > 0;
   const int n_unocc = run_unocc ? gpp::unocc_grid(dp, tile) : 0;
   const int n_occ = run_occ ? gpp::occ_grid(dp) : 0;
   const int n_ach = want_ach ? gpp::achstemp_records(dp, o->semantics) : 0;
   const int nrec = n_unocc + n_occ;
-  if ((rc = ensure(h, h->part,
-                   (size_t)(nrec + n_ach + 1) * gpp::kSlots * sizeof(double))))
+  // h->part: [counter 64 B][records ...][2 result slots]
+  const size_t rec_bytes = (size_t)(nrec + n_ach) * gpp::kSlots * 8;
+  if ((rc = ensure(h, h->part, 256 + rec_bytes + 2 * gpp::kSlots * 8)))
     return rc;
-  double *part = (double *)h->part.ptr;
-  double *dout = (double *)h->out.ptr;
-
-  if (o->precision == GPP_PREC_MIXED && run_unocc) {
-    if ((rc = ensure(h, h->aqsn32, nbnc * sizeof(float2)))) return rc;
-    dp.aqsn32 = (const float2 *)h->aqsn32.ptr;
+  unsigned long long *dcount = counter_ptr(h);
+  double *part = (double *)((char *)h->part.ptr + 256);
+  double *dout = part + (size_t)(nrec + n_ach) * gpp::kSlots;
+  double *n1u = nullptr, *n1o = nullptr, *n1out = nullptr;
+  if (want_n1) {
+    const size_t nu = run_unocc ? (size_t)n_unocc * (NB - n1u_begin) : 0;
+    const size_t no = run_occ ? (size_t)n_occ * n1o_end : 0;
+    if ((rc = ensure(h, h->n1log, (nu + no + NB + 1) * 16))) return rc;
+    n1u = (double *)h->n1log.ptr;
+    n1o = n1u + 2 * nu;
+    n1out = n1o + 2 * no;
   }
+  if ((rc = ensure_host_out(h, 2 * gpp::kSlots + 2 * (size_t)NB + 2)))
+    return rc;
 
   HIP_OR_RET(h, hipEventRecord(h->ev[0], s));
-  HIP_OR_RET(h, gpp::launch_prepass((const int *)dev[5], (const int *)dev[6],
-                                    (const double *)dev[7], p->wx_per_band,
-                                    NB, NC, NG, (int *)h->igp_of.ptr,
-                                    (double *)h->wx.ptr,
-                                    (unsigned long long *)h->count.ptr, s));
-  if (o->precision == GPP_PREC_MIXED && run_unocc)
-    HIP_OR_RET(h, gpp::launch_aqsn_to_fp32(dp.aqsn,
-                                           (float2 *)h->aqsn32.ptr, nbnc, s));
-  unsigned long long *dcount = (unsigned long long *)h->count.ptr;
+  if ((rc = enqueue_prepass(h, prep[0], mixed, dcount, s))) return rc;
   HIP_OR_RET(h, hipEventRecord(h->ev[1], s));
   if (run_unocc) {
     const double thr =
-        o->resonance_threshold > 0 ? o->resonance_threshold : 1e-2;
-    HIP_OR_RET(h, gpp::launch_unocc(dp, n1_unocc_begin, NB, o->precision,
-                                    o->semantics, tile, thr, part,
-                                    o->precision == GPP_PREC_MIXED ? dcount
-                                                                   : nullptr,
-                                    s));
+        o->resonance_threshold > 0 ? o->resonance_threshold : kDefaultResonance;
+    HIP_OR_RET(h, gpp::launch_unocc(dp, n1u_begin, NB, o->precision,
+                                    o->semantics, tile, thr, part, n1u,
+                                    mixed ? dcount : nullptr, s));
   }
   HIP_OR_RET(h, hipEventRecord(h->ev[2], s));
   if (run_occ)
-    HIP_OR_RET(h, gpp::launch_occ(dp, n1_occ_end, o->semantics,
-                                  part + (size_t)n_unocc * gpp::kSlots, s));
+    HIP_OR_RET(h, gpp::launch_occ(dp, n1o_end, o->semantics,
+                                  part + (size_t)n_unocc * gpp::kSlots, n1o,
+                                  s));
   HIP_OR_RET(h, hipEventRecord(h->ev[3], s));
   if (n_ach)
     HIP_OR_RET(h, gpp::launch_achstemp(dp, o->semantics,
@@ -328,24 +510,27 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   HIP_OR_RET(h, gpp::launch_reduce(part, nrec, dout, s));
   HIP_OR_RET(h, gpp::launch_reduce(part + (size_t)nrec * gpp::kSlots, n_ach,
                                    dout + gpp::kSlots, s));
+  if (want_n1)
+    HIP_OR_RET(h, gpp::launch_reduce_n1(n1u, run_unocc ? n_unocc : 0,
+                                        n1u_begin,
+                                        run_unocc ? NB - n1u_begin : 0, n1o,
+                                        run_occ ? n_occ : 0, n1o_end, NB,
+                                        n1out, s));
   HIP_OR_RET(h, hipEventRecord(h->ev[4], s));
-  HIP_OR_RET(h, hipMemcpyAsync(h->host_out, dout,
-                               2 * gpp::kSlots * sizeof(double),
+  double *ho = h->host_out;
+  HIP_OR_RET(h, hipMemcpyAsync(ho, dout, 2 * gpp::kSlots * sizeof(double),
                                hipMemcpyDeviceToHost, s));
-  if (o->precision == GPP_PREC_MIXED)
-    HIP_OR_RET(h, hipMemcpyAsync(h->host_count, dcount, sizeof *dcount,
-                                 hipMemcpyDeviceToHost, s));
+  if (want_n1 && NB > 0)
+    HIP_OR_RET(h, hipMemcpyAsync(ho + 2 * gpp::kSlots, n1out,
+                                 (size_t)NB * 16, hipMemcpyDeviceToHost, s));
+  if (mixed)
+    HIP_OR_RET(h, hipMemcpyAsync(ho + 2 * gpp::kSlots + 2 * (size_t)NB, dcount,
+                                 8, hipMemcpyDeviceToHost, s));
   HIP_OR_RET(h, hipStreamSynchronize(s));
 
-  const double *ho = h->host_out;
-  for (int iw = 0; iw < 3; ++iw) {
-    r->achtemp[iw].re = ho[2 * iw];
-    r->achtemp[iw].im = ho[2 * iw + 1];
-    r->asxtemp[iw].re = ho[6 + 2 * iw];
-    r->asxtemp[iw].im = ho[6 + 2 * iw + 1];
-  }
-  r->achstemp.re = ho[gpp::kSlots + 0];
-  r->achstemp.im = ho[gpp::kSlots + 1];
+  fill_result(ho, ho + gpp::kSlots, r);
+  if (want_n1)
+    std::memcpy(r->acht_n1_loc, ho + 2 * gpp::kSlots, (size_t)NB * 16);
   float ms = 0.f;
   HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
   r->unocc_kernel_ms = ms;
@@ -353,13 +538,10 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   r->occ_kernel_ms = ms;
   HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[4]));
   r->total_ms = ms;
-  r->terms = (double)NB * NG * (double)NC * 3.0;
-  r->fp64_fallback_terms =
-      o->precision == GPP_PREC_MIXED ? (long long)*h->host_count : 0;
-  if (r->acht_n1_loc) {
-    // Per-band log is not produced by the fused kernels yet.
-    return set_err(h, GPP_EUNSUPPORTED, "acht_n1_loc output not supported");
-  }
+  r->terms = (double)NB * dp.NG * (double)dp.NC * 3.0;
+  long long fb = 0;
+  if (mixed) std::memcpy(&fb, ho + 2 * gpp::kSlots + 2 * (size_t)NB, 8);
+  r->fp64_fallback_terms = fb;
   if (o->flags & GPP_FLAG_CHECK_FINITE) {
     for (int i = 0; i < 12; ++i)
       if (!std::isfinite(ho[i]))
@@ -368,14 +550,276 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   return GPP_OK;
 }
 
+int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *o,
+                 int *tile_ig, int *waves_per_block) {
+  if (!h) return GPP_EINVAL;
+  if (!tile_ig || !waves_per_block)
+    return set_err(h, GPP_EINVAL, "null output");
+  int rc = validate(h, p, o);
+  if (rc) return rc;
+  // Time the non-occupied kernel on a slice of at most 32 unoccupied bands
+  // (cost is linear in bands) for every candidate tile.
+  gpp_problem q = *p;
+  const int nv = std::min(p->nvband, p->number_bands);
+  const int nb = std::min(p->number_bands - nv, 32);
+  const TuneKey key = tune_key(p, o);
+  TileCfg best = default_tile(o->precision);
+  if (nb > 0 && p->ngpown > 0) {
+    q.number_bands = nb;
+    q.nvband = 0;
+    q.aqsmtemp = p->aqsmtemp + (size_t)nv * p->ncouls;
+    q.aqsntemp = p->aqsntemp + (size_t)nv * p->ncouls;
+    if (p->wx_per_band) q.wx_array = p->wx_array + 3 * (size_t)nv;
+    gpp_options to = *o;
+    to.flags = GPP_FLAG_NO_ACHSTEMP;
+    const int ks_fp64[] = {4, 8, 16};
+    const int ks_mixed[] = {8, 16, 32};
+    const int *ks = o->precision == GPP_PREC_MIXED ? ks_mixed : ks_fp64;
+    const int ws[] = {1, 2, 4};
+    double best_ms = 1e300;
+    for (int i = 0; i < 3; ++i)
+      for (int w : ws) {
+        to.tile_ig = ks[i];
+        to.waves_per_block = w;
+        gpp_result r;
+        std::memset(&r, 0, sizeof r);
+        double t = 1e300;
+        for (int rep = 0; rep < 3; ++rep) {
+          if ((rc = gpp_sigma_ex(h, &q, &to, &r))) return rc;
+          if (rep > 0) t = std::min(t, r.unocc_kernel_ms);
+        }
+        if (t < best_ms) {
+          best_ms = t;
+          best.k_ig = ks[i];
+          best.waves = w;
+        }
+      }
+  }
+  h->tuned[key] = best;
+  save_tune_entry(h, key, best);
+  *tile_ig = best.k_ig;
+  *waves_per_block = best.waves;
+  return GPP_OK;
+}
+
 int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
                     const gpp_options *o, gpp_result *r) {
   if (!h) return GPP_EINVAL;
-  if (n < 0 || (n > 0 && (!p || !r)))
+  h->err.clear();
+  if (n < 0 || (n > 0 && (!p || !r)) || !o)
     return set_err(h, GPP_EINVAL, "bad batch arguments");
+  if (n == 0) return GPP_OK;
+  int rc;
   for (int i = 0; i < n; ++i) {
-    int rc = gpp_sigma_ex(h, p + i, o, r + i);
-    if (rc) return rc;
+    if ((rc = validate(h, p + i, o))) return rc;
+    if (r[i].acht_n1_loc)
+      return set_err(h, GPP_EUNSUPPORTED,
+                     "acht_n1_loc is not produced by the batched launch");
+  }
+  if (o->flags & (GPP_FLAG_ALL_BANDS_UNOCC | GPP_FLAG_SKIP_UNOCC))
+    return set_err(h, GPP_EUNSUPPORTED,
+                   "band-range flags are not supported in batches");
+  if (o->waves_per_block && o->waves_per_block != 4)
+    return set_err(h, GPP_EINVAL, "batched launch uses 4 waves per block");
+  HIP_OR_RET(h, hipSetDevice(h->device));
+  hipStream_t s = (o->memory == GPP_MEM_DEVICE && o->stream)
+                      ? (hipStream_t)o->stream
+                      : h->stream;
+  const bool mixed = o->precision == GPP_PREC_MIXED;
+  const bool want_ach = !(o->flags & GPP_FLAG_NO_ACHSTEMP);
+
+  // Tile: explicit, else tuned on the largest problem, else default.
+  int big = 0;
+  for (int i = 1; i < n; ++i)
+    if ((double)p[i].number_bands * p[i].ngpown * p[i].ncouls >
+        (double)p[big].number_bands * p[big].ngpown * p[big].ncouls)
+      big = i;
+  TileCfg tile = default_tile(o->precision);
+  if (o->tile_ig) {
+    tile.k_ig = o->tile_ig;
+  } else {
+    auto it = h->tuned.find(tune_key(p + big, o));
+    if (it == h->tuned.end() && (o->flags & GPP_FLAG_AUTOTUNE)) {
+      int k, w;
+      if ((rc = gpp_autotune(h, p + big, o, &k, &w))) return rc;
+      it = h->tuned.find(tune_key(p + big, o));
+    }
+    if (it != h->tuned.end()) tile.k_ig = it->second.k_ig;
+  }
+  tile.waves = 4;
+
+  std::vector<Prepared> prep;
+  if ((rc = prepare(h, p, n, o, mixed, s, prep))) return rc;
+
+  // Work list, records and cost model (terms x per-term cost weight).
+  std::vector<BatchItem> items;
+  std::vector<double> cost;
+  std::vector<int> rec_begin(n), rec_count(n), ach_begin(n), ach_count(n);
+  int nrec = 0;
+  for (int i = 0; i < n; ++i) {
+    const DevProblem &dp = prep[i].dp;
+    rec_begin[i] = nrec;
+    const int nu = dp.NB > dp.NV ? gpp::unocc_grid(dp, tile) : 0;
+    const int no = dp.NV > 0 ? gpp::occ_grid(dp) : 0;
+    const double cu = (double)(dp.NB - dp.NV) * tile.k_ig * 4 * 64;
+    const double co = (double)dp.NV * std::min(dp.NC, 2048) * 5.0;
+    for (int b = 0; b < nu; ++b) {
+      items.push_back(BatchItem{i, 0, b, nrec++});
+      cost.push_back(cu);
+    }
+    for (int b = 0; b < no; ++b) {
+      items.push_back(BatchItem{i, 1, b, nrec++});
+      cost.push_back(co);
+    }
+    rec_count[i] = nu + no;
+  }
+  {
+    std::vector<int> order(items.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<BatchItem> sorted(items.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
+    items.swap(sorted);
+  }
+  int nach = 0;
+  for (int i = 0; i < n; ++i) {
+    ach_begin[i] = nrec + nach;
+    ach_count[i] =
+        want_ach ? gpp::achstemp_records(prep[i].dp, o->semantics) : 0;
+    nach += ach_count[i];
+  }
+
+  // Device metadata: problem table, items, record ranges.
+  std::vector<DevProblem> table(n);
+  for (int i = 0; i < n; ++i) table[i] = prep[i].dp;
+  Arena ma;
+  const size_t off_tab = ma.take(sizeof(DevProblem) * n);
+  const size_t off_items = ma.take(sizeof(BatchItem) * items.size());
+  const size_t off_rb = ma.take(sizeof(int) * n);
+  const size_t off_rc = ma.take(sizeof(int) * n);
+  const size_t off_ab = ma.take(sizeof(int) * n);
+  const size_t off_ac = ma.take(sizeof(int) * n);
+  if ((rc = ensure(h, h->meta, ma.used))) return rc;
+  char *mb = (char *)h->meta.ptr;
+  std::vector<unsigned char> meta_host(ma.used, 0);
+  std::memcpy(meta_host.data() + off_tab, table.data(), sizeof(DevProblem) * n);
+  std::memcpy(meta_host.data() + off_items, items.data(),
+              sizeof(BatchItem) * items.size());
+  std::memcpy(meta_host.data() + off_rb, rec_begin.data(), sizeof(int) * n);
+  std::memcpy(meta_host.data() + off_rc, rec_count.data(), sizeof(int) * n);
+  std::memcpy(meta_host.data() + off_ab, ach_begin.data(), sizeof(int) * n);
+  std::memcpy(meta_host.data() + off_ac, ach_count.data(), sizeof(int) * n);
+
+  // h->part: [counters 256 B][records][n x 2 result slots]
+  const size_t rec_bytes = (size_t)(nrec + nach) * gpp::kSlots * 8;
+  if ((rc = ensure(h, h->part,
+                   256 + rec_bytes + (size_t)2 * n * gpp::kSlots * 8)))
+    return rc;
+  unsigned long long *dcount = counter_ptr(h);
+  int *work_counter = (int *)((char *)h->part.ptr + 64);
+  double *part = (double *)((char *)h->part.ptr + 256);
+  double *dout = part + (size_t)(nrec + nach) * gpp::kSlots;
+  double *dout_ach = dout + (size_t)n * gpp::kSlots;
+  if ((rc = ensure_host_out(h, (size_t)2 * n * gpp::kSlots + 1))) return rc;
+
+  const int per_cu = gpp::batch_blocks_per_cu(tile.k_ig, o->precision,
+                                              o->semantics);
+  hipDeviceProp_t prop;
+  HIP_OR_RET(h, hipGetDeviceProperties(&prop, h->device));
+  int grid = prop.multiProcessorCount * per_cu;
+  if (grid > (int)items.size()) grid = (int)items.size();
+  if (grid < 1) grid = 1;
+  const double thr =
+      o->resonance_threshold > 0 ? o->resonance_threshold : kDefaultResonance;
+
+  // Graph signature: everything that determines the launch sequence.
+  std::vector<unsigned char> sig(meta_host);
+  auto push = [&](const void *v, size_t b) {
+    const unsigned char *c = (const unsigned char *)v;
+    sig.insert(sig.end(), c, c + b);
+  };
+  push(o, sizeof *o);
+  push(&tile, sizeof tile);
+  push(&grid, sizeof grid);
+  void *bases[4] = {h->part.ptr, h->meta.ptr, h->work.ptr, (void *)s};
+  push(bases, sizeof bases);
+
+  HIP_OR_RET(h, hipMemcpyAsync(mb, meta_host.data(), ma.used,
+                               hipMemcpyHostToDevice, s));
+  HIP_OR_RET(h, hipEventRecord(h->ev[0], s));
+  if (!(h->graph_exec && h->graph_sig == sig)) {
+    if (h->graph_exec) {
+      (void)hipGraphExecDestroy(h->graph_exec);
+      h->graph_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    HIP_OR_RET(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int crc = GPP_OK;
+    for (int i = 0; i < n && crc == GPP_OK; ++i)
+      crc = enqueue_prepass(h, prep[i], mixed, dcount, s);
+    if (crc == GPP_OK) {
+      hipError_t e = gpp::launch_batch(
+          (const DevProblem *)(mb + off_tab),
+          (const BatchItem *)(mb + off_items), (int)items.size(), tile.k_ig,
+          o->precision, o->semantics, thr, part, work_counter, dcount, grid,
+          (const int *)(mb + off_rb), (const int *)(mb + off_rc), n, dout, s);
+      if (e != hipSuccess)
+        crc = set_err(h, GPP_EDEVICE, "batch launch: %s", hipGetErrorString(e));
+    }
+    for (int i = 0; i < n && crc == GPP_OK && want_ach; ++i) {
+      hipError_t e = gpp::launch_achstemp(
+          prep[i].dp, o->semantics, part + (size_t)ach_begin[i] * gpp::kSlots,
+          s);
+      if (e != hipSuccess)
+        crc = set_err(h, GPP_EDEVICE, "achstemp: %s", hipGetErrorString(e));
+    }
+    if (crc == GPP_OK && want_ach) {
+      hipError_t e = gpp::launch_reduce_batch(
+          part, (const int *)(mb + off_ab), (const int *)(mb + off_ac), n,
+          dout_ach, s);
+      if (e != hipSuccess)
+        crc = set_err(h, GPP_EDEVICE, "achstemp reduce: %s",
+                      hipGetErrorString(e));
+    }
+    hipError_t ee = hipStreamEndCapture(s, &g);
+    if (crc != GPP_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return crc;
+    }
+    HIP_OR_RET(h, ee);
+    hipError_t ie = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIP_OR_RET(h, ie);
+    h->graph_sig.swap(sig);
+  }
+  HIP_OR_RET(h, hipGraphLaunch(h->graph_exec, s));
+  HIP_OR_RET(h, hipEventRecord(h->ev[1], s));
+  double *ho = h->host_out;
+  HIP_OR_RET(h, hipMemcpyAsync(ho, dout, (size_t)2 * n * gpp::kSlots * 8,
+                               hipMemcpyDeviceToHost, s));
+  HIP_OR_RET(h, hipMemcpyAsync(ho + (size_t)2 * n * gpp::kSlots, dcount, 8,
+                               hipMemcpyDeviceToHost, s));
+  HIP_OR_RET(h, hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+  long long fb = 0;
+  std::memcpy(&fb, ho + (size_t)2 * n * gpp::kSlots, 8);
+  for (int i = 0; i < n; ++i) {
+    fill_result(ho + (size_t)i * gpp::kSlots,
+                ho + (size_t)(n + i) * gpp::kSlots, r + i);
+    if (!want_ach) r[i].achstemp.re = r[i].achstemp.im = 0.0;
+    r[i].unocc_kernel_ms = ms;
+    r[i].occ_kernel_ms = 0.0;
+    r[i].total_ms = ms;
+    r[i].terms = (double)p[i].number_bands * p[i].ngpown *
+                 (double)p[i].ncouls * 3.0;
+    r[i].fp64_fallback_terms = (i == 0) ? fb : 0;
+    if (o->flags & GPP_FLAG_CHECK_FINITE)
+      for (int j = 0; j < 12; ++j)
+        if (!std::isfinite(ho[(size_t)i * gpp::kSlots + j]))
+          return set_err(h, GPP_ENONFINITE,
+                         "non-finite result in problem %d slot %d", i, j);
   }
   return GPP_OK;
 }

@@ -70,6 +70,34 @@ __device__ __forceinline__ void block_store(double (&v)[N], double *rec) {
   }
 }
 
+// Per-band log (acht_n1_loc): sum one complex value per lane over the block
+// (lanes by butterfly, waves in order) and store it at dst[2*idx].  Called
+// by every thread of the block; two barriers.
+__device__ __forceinline__ void n1log_store(double re, double im, double *dst,
+                                            size_t idx) {
+  __shared__ double lg[16][2];
+  re = wave_sum(re);
+  im = wave_sum(im);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+  if (lane == 0) {
+    lg[wave][0] = re;
+    lg[wave][1] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sr = 0.0, si = 0.0;
+    for (int w = 0; w < nw; ++w) {
+      sr += lg[w][0];
+      si += lg[w][1];
+    }
+    dst[2 * idx] = sr;
+    dst[2 * idx + 1] = si;
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // Prepass: resolve igp per my_igp and expand/clamp the frequency grid.
 // Index rule of gppKer.cpp:287-293: indigp = inv_igp_index[my_igp];
@@ -128,11 +156,13 @@ __global__ void to_fp32_kernel(const double2 *__restrict__ in,
 //   S  += Y / |wdiff|^2              (per iw, over the K G-vectors)
 // and after the run: acc += conj(aqsm[n1][igp]) * S.
 // Equals conj(aqsm)*aqsn*delw*I_eps of gppKer.cpp:328-337 term by term.
-template <int K, bool MASK>
-__global__ __launch_bounds__(256) void unocc_fp64_kernel(
-    DevProblem p, int n1_begin, int n1_end, double *__restrict__ partials,
-    int rec_base, int block_offset) {
-  const int b = blockIdx.x + block_offset;
+// n1log (optional): [block][n1 - n1_begin] complex, the block's share of
+// acht_n1_loc (gppKer.cpp:357).
+template <int K, bool MASK, bool N1LOG>
+__device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
+                                                 int n1_begin, int n1_end,
+                                                 int b, double *__restrict__ rec,
+                                                 double *__restrict__ n1log) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = uniform(threadIdx.x / kWave);
   const int W = blockDim.x / kWave;
@@ -171,6 +201,8 @@ __global__ __launch_bounds__(256) void unocc_fp64_kernel(
     wt2[k] = wr[k] * wr[k] + wi2[k];
   }
 
+  // sch_array[iw] = 0.5 * sum; achtemp[iw] += sch_array[iw] * vcoul[igp]
+  const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double acc_r[3] = {0.0, 0.0, 0.0}, acc_i[3] = {0.0, 0.0, 0.0};
   const double2 *aqsm_col = p.aqsm + igp_c;
   double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NC]
@@ -262,19 +294,29 @@ __global__ __launch_bounds__(256) void unocc_fp64_kernel(
     acc_i[1] = fma(am.x, s_i1, fma(-am.y, s_r1, acc_i[1]));
     acc_r[2] = fma(am.x, s_r2, fma(am.y, s_i2, acc_r[2]));
     acc_i[2] = fma(am.x, s_i2, fma(-am.y, s_r2, acc_i[2]));
+    if (N1LOG)  // acht_n1_loc[n1] += sch_array[2] * vcoul[igp]
+      n1log_store(wgt * fma(am.x, s_r2, am.y * s_i2),
+                  wgt * fma(am.x, s_i2, -am.y * s_r2), n1log,
+                  (size_t)b * (n1_end - n1_begin) + (n1 - n1_begin));
   }
 
-  // sch_array[iw] = 0.5 * sum; achtemp[iw] += sch_array[iw] * vcoul[igp]
-  const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double v[6];
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {
     v[2 * iw] = wgt * acc_r[iw];
     v[2 * iw + 1] = wgt * acc_i[iw];
   }
-  double *rec = partials + (size_t)(rec_base + b) * kSlots;
   block_store<6>(v, rec);
   if (threadIdx.x >= 6 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
+}
+
+template <int K, bool MASK, bool N1LOG>
+__global__ __launch_bounds__(256) void unocc_fp64_kernel(
+    DevProblem p, int n1_begin, int n1_end, double *__restrict__ partials,
+    double *__restrict__ n1log) {
+  unocc_fp64_block<K, MASK, N1LOG>(p, n1_begin, n1_end, blockIdx.x,
+                                   partials + (size_t)blockIdx.x * kSlots,
+                                   n1log);
 }
 
 // ---------------------------------------------------------------------------
@@ -285,12 +327,11 @@ __global__ __launch_bounds__(256) void unocc_fp64_kernel(
 // |wdiff|^2 < thresh are re-evaluated fully in FP64 (from the FP64 arrays)
 // and accumulated in FP64.  The per-band run sum S is FP32 over K terms;
 // everything from conj(aqsm)*S on is FP64.
-template <int K>
-__global__ __launch_bounds__(256) void unocc_mixed_kernel(
-    DevProblem p, int n1_begin, int n1_end, double *__restrict__ partials,
-    int rec_base, int block_offset, float thresh,
+template <int K, bool N1LOG>
+__device__ __forceinline__ void unocc_mixed_block(
+    const DevProblem &p, int n1_begin, int n1_end, int b,
+    double *__restrict__ rec, double *__restrict__ n1log, float thresh,
     unsigned long long *__restrict__ fallback_count) {
-  const int b = blockIdx.x + block_offset;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = uniform(threadIdx.x / kWave);
   const int W = blockDim.x / kWave;
@@ -319,7 +360,9 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
       Ar[k] = (float)(w.x * e.x - w.y * e.y);
       Ai[k] = (float)(w.x * e.y + w.y * e.x);
     } else {
-      wr[k] = 0.0;
+      // Inert slot: A = 0 and a far-away wtilde, so |wdiff|^2 overflows to
+      // +inf in FP32 (contribution 0, never routed to the FP64 redo).
+      wr[k] = 1e30;
       wi[k] = 0.0f;
       Ar[k] = 0.0f;
       Ai[k] = 0.0f;
@@ -327,6 +370,7 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
     wi2[k] = wi[k] * wi[k];
   }
 
+  const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double acc_r[3] = {0.0, 0.0, 0.0}, acc_i[3] = {0.0, 0.0, 0.0};
   unsigned long long nfb = 0;
   const double2 *aqsm_col = p.aqsm + igp_c;
@@ -335,13 +379,14 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
 
   for (int n1 = n1_begin; n1 < n1_end; ++n1) {
     const double2 am = am_next;
-    if (n1 + 1 < n1_end) am_next = aqsm_col[(size_t)(n1 + 1) * NC];
+    am_next = aqsm_col[(size_t)min(n1 + 1, n1_end - 1) * NC];
     const double x[3] = {p.wx[n1 * 3 + 0], p.wx[n1 * 3 + 1],
                          p.wx[n1 * 3 + 2]};
     const float2 *an_row = p.aqsn32 + (size_t)n1 * NC;
 
     float s_r[3] = {0.f, 0.f, 0.f}, s_i[3] = {0.f, 0.f, 0.f};
     double f_r[3] = {0.0, 0.0, 0.0}, f_i[3] = {0.0, 0.0, 0.0};
+    bool bad = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int igk = min(ig0 + k, NC - 1);
@@ -354,46 +399,64 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
       for (int iw = 0; iw < 3; ++iw) {
         const float dr = (float)(x[iw] - wr[k]);
         const float d = fmaf(dr, dr, wi2[k]);
-        float r = rcp_f32(d);
-        if (__builtin_expect(!(d >= thresh), 0)) {
-          // Near resonance (or NaN): redo this term in FP64.
-          r = 0.0f;
-          const size_t off = (size_t)my_igp * NC + (ig0 + k);
-          const double2 w = p.wtilde[off];
-          const double2 e = p.ieps[off];
-          const double2 a = p.aqsn[(size_t)n1 * NC + igk];
-          const double A_r = w.x * e.x - w.y * e.y;
-          const double A_i = w.x * e.y + w.y * e.x;
-          const double B_r = a.x * A_r - a.y * A_i;
-          const double B_i = a.x * A_i + a.y * A_r;
-          const double ddr = x[iw] - w.x;
-          const double dd = ddr * ddr + w.y * w.y;
-          const double rr = 1.0 / dd;
-          f_r[iw] = fma((B_r * ddr - B_i * w.y), rr, f_r[iw]);
-          f_i[iw] = fma((B_r * w.y + B_i * ddr), rr, f_i[iw]);
-          ++nfb;
-        }
+        // Near resonance (or NaN): excluded here, redone in FP64 below.
+        const bool nr = !(d >= thresh);
+        bad |= nr;
+        const float r = nr ? 0.0f : rcp_f32(d);
         s_r[iw] = fmaf(fmaf(Br, dr, nBiwi), r, s_r[iw]);
         s_i[iw] = fmaf(fmaf(Bi, dr, Brwi), r, s_i[iw]);
       }
     }
+    if (__builtin_expect(bad, 0)) {
+      // The same FP32 test picks exactly the terms the loop above left out.
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int iw = 0; iw < 3; ++iw) {
+          const float dr = (float)(x[iw] - wr[k]);
+          const float d = fmaf(dr, dr, wi2[k]);
+          if (!(d >= thresh) && valid && ig0 + k < NC) {
+            const size_t off = (size_t)my_igp * NC + (ig0 + k);
+            const double2 w = p.wtilde[off];
+            const double2 e = p.ieps[off];
+            const double2 a = p.aqsn[(size_t)n1 * NC + (ig0 + k)];
+            const double A_r = w.x * e.x - w.y * e.y;
+            const double A_i = w.x * e.y + w.y * e.x;
+            const double B_r = a.x * A_r - a.y * A_i;
+            const double B_i = a.x * A_i + a.y * A_r;
+            const double ddr = x[iw] - w.x;
+            const double dd = ddr * ddr + w.y * w.y;
+            const double rr = 1.0 / dd;
+            f_r[iw] = fma((B_r * ddr - B_i * w.y), rr, f_r[iw]);
+            f_i[iw] = fma((B_r * w.y + B_i * ddr), rr, f_i[iw]);
+            ++nfb;
+          }
+        }
+      }
+    }
+    double lr = 0.0, li = 0.0;
 #pragma unroll
     for (int iw = 0; iw < 3; ++iw) {
       const double sr = (double)s_r[iw] + f_r[iw];
       const double si = (double)s_i[iw] + f_i[iw];
       acc_r[iw] = fma(am.x, sr, fma(am.y, si, acc_r[iw]));
       acc_i[iw] = fma(am.x, si, fma(-am.y, sr, acc_i[iw]));
+      if (iw == 2) {
+        lr = wgt * fma(am.x, sr, am.y * si);
+        li = wgt * fma(am.x, si, -am.y * sr);
+      }
     }
+    if (N1LOG)
+      n1log_store(lr, li, n1log,
+                  (size_t)b * (n1_end - n1_begin) + (n1 - n1_begin));
   }
 
-  const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double v[6];
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {
     v[2 * iw] = wgt * acc_r[iw];
     v[2 * iw + 1] = wgt * acc_i[iw];
   }
-  double *rec = partials + (size_t)(rec_base + b) * kSlots;
   block_store<6>(v, rec);
   if (threadIdx.x >= 6 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
   if (fallback_count) {
@@ -405,46 +468,66 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
   }
 }
 
+template <int K, bool N1LOG>
+__global__ __launch_bounds__(256) void unocc_mixed_kernel(
+    DevProblem p, int n1_begin, int n1_end, double *__restrict__ partials,
+    double *__restrict__ n1log, float thresh,
+    unsigned long long *__restrict__ fallback_count) {
+  unocc_mixed_block<K, N1LOG>(p, n1_begin, n1_end, blockIdx.x,
+                              partials + (size_t)blockIdx.x * kSlots, n1log,
+                              thresh, fallback_count);
+}
+
 // ---------------------------------------------------------------------------
 // Occupied bands (n1 < nvband): ssxt_scht_solver / flagOCC_solver of
 // gppKer.cpp:16-62,122-146 (SEM gppker, gppker-strict) or the occupied
 // branch of gppkernel.deslippe.09.30.2014.f90:271-334 (SEM bgw).
+// Block b: my_igp = b / nch, G-vector chunk b % nch; threads stride over the
+// chunk.  Bands outer so that the per-band log needs no per-band registers.
 constexpr int kOccThreads = 256;
 constexpr int kOccChunk = 2048;  // G-vectors per block
 
-template <int SEM>
-__global__ __launch_bounds__(kOccThreads) void occ_kernel(
-    DevProblem p, int n1_end, double *__restrict__ partials, int rec_base) {
+template <int SEM, bool N1LOG>
+__device__ __forceinline__ void occ_block(const DevProblem &p, int n1_end,
+                                          int b, double *__restrict__ rec,
+                                          double *__restrict__ n1log) {
   const int NC = p.NC, NG = p.NG;
   const int nch = (NC + kOccChunk - 1) / kOccChunk;
-  const int my_igp = blockIdx.x / nch;
-  const int ch = blockIdx.x % nch;
+  const int my_igp = b / nch;
+  const int ch = b % nch;
   const int igp = p.igp_of[my_igp];
-  double *rec = partials + (size_t)(rec_base + blockIdx.x) * kSlots;
   if (igp < 0) {  // skipped row (block-uniform)
     if (threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
+    if (N1LOG)
+      for (int n1 = 0; n1 < n1_end; ++n1)
+        n1log_store(0.0, 0.0, n1log, (size_t)b * n1_end + n1);
     return;
   }
+  const double vc = p.vcoul[igp];
+  const double wsch = 0.5 * vc;                              // :310,355
+  const double wssx = (SEM == kSemBgw ? -kOcc : kOcc) * vc;  // :352 / f90:393
   double sch_r[3] = {0, 0, 0}, sch_i[3] = {0, 0, 0};
   double ssx_r[3] = {0, 0, 0}, ssx_i[3] = {0, 0, 0};
   const int ig_end = min(NC, (ch + 1) * kOccChunk);
-  for (int ig = ch * kOccChunk + threadIdx.x; ig < ig_end;
-       ig += kOccThreads) {
-    const size_t off = (size_t)my_igp * NC + ig;
-    const double2 w = p.wtilde[off];
-    const double2 e = p.ieps[off];
-    // I_eps used for sch and for the cutoff.  gppKer.cpp:139 passes
-    // I_eps_array[my_igp*ngpown+ig]; the strict mode keeps that index.
-    const double2 ex =
-        (SEM == kSemGppkerStrict) ? p.ieps[(size_t)my_igp * NG + ig] : e;
-    // wtilde2 = wtilde^2, Omega2 = wtilde2 * I_eps (gppKer.cpp:132-133)
-    const double w2r = w.x * w.x - w.y * w.y;
-    const double w2i = 2.0 * w.x * w.y;
-    const double omr = w2r * e.x - w2i * e.y;
-    const double omi = w2r * e.y + w2i * e.x;
-    const double cut2 = kSexcut * kSexcut * (ex.x * ex.x + ex.y * ex.y);
-    for (int n1 = 0; n1 < n1_end; ++n1) {
-      const double2 am = p.aqsm[(size_t)n1 * NC + igp];
+  for (int n1 = 0; n1 < n1_end; ++n1) {
+    const double2 am = p.aqsm[(size_t)n1 * NC + igp];
+    double tsr[3] = {0, 0, 0}, tsi[3] = {0, 0, 0};
+    double txr[3] = {0, 0, 0}, txi[3] = {0, 0, 0};
+    for (int ig = ch * kOccChunk + threadIdx.x; ig < ig_end;
+         ig += blockDim.x) {
+      const size_t off = (size_t)my_igp * NC + ig;
+      const double2 w = p.wtilde[off];
+      const double2 e = p.ieps[off];
+      // I_eps used for sch and for the cutoff.  gppKer.cpp:139 passes
+      // I_eps_array[my_igp*ngpown+ig]; the strict mode keeps that index.
+      const double2 ex =
+          (SEM == kSemGppkerStrict) ? p.ieps[(size_t)my_igp * NG + ig] : e;
+      // wtilde2 = wtilde^2, Omega2 = wtilde2 * I_eps (gppKer.cpp:132-133)
+      const double w2r = w.x * w.x - w.y * w.y;
+      const double w2i = 2.0 * w.x * w.y;
+      const double omr = w2r * e.x - w2i * e.y;
+      const double omi = w2r * e.y + w2i * e.x;
+      const double cut2 = kSexcut * kSexcut * (ex.x * ex.x + ex.y * ex.y);
       const double2 an = p.aqsn[(size_t)n1 * NC + ig];
       // matngmatmgp = aqsn[n1][ig] * conj(aqsm[n1][igp])
       const double mr = an.x * am.x + an.y * am.y;
@@ -497,16 +580,23 @@ __global__ __launch_bounds__(kOccThreads) void occ_kernel(
           }
         }
         // (gppKer.cpp:58 tests abs(wxt) < 0, which never holds.)
-        ssx_r[iw] += mr * ssr - mi * ssi;
-        ssx_i[iw] += mr * ssi + mi * ssr;
-        sch_r[iw] += mr * schr - mi * schi;
-        sch_i[iw] += mr * schi + mi * schr;
+        txr[iw] += mr * ssr - mi * ssi;
+        txi[iw] += mr * ssi + mi * ssr;
+        tsr[iw] += mr * schr - mi * schi;
+        tsi[iw] += mr * schi + mi * schr;
       }
     }
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      sch_r[iw] += tsr[iw];
+      sch_i[iw] += tsi[iw];
+      ssx_r[iw] += txr[iw];
+      ssx_i[iw] += txi[iw];
+    }
+    if (N1LOG)  // acht_n1_loc[n1] += sch_array[2] * vcoul[igp]
+      n1log_store(wsch * tsr[2], wsch * tsi[2], n1log,
+                  (size_t)b * n1_end + n1);
   }
-  const double vc = p.vcoul[igp];
-  const double wsch = 0.5 * vc;                                // :310,355
-  const double wssx = (SEM == kSemBgw ? -kOcc : kOcc) * vc;    // :352 / f90:393
   double v[12];
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {
@@ -516,6 +606,53 @@ __global__ __launch_bounds__(kOccThreads) void occ_kernel(
     v[6 + 2 * iw + 1] = wssx * ssx_i[iw];
   }
   block_store<12>(v, rec);
+}
+
+template <int SEM, bool N1LOG>
+__global__ __launch_bounds__(kOccThreads) void occ_kernel(
+    DevProblem p, int n1_end, double *__restrict__ partials,
+    double *__restrict__ n1log) {
+  occ_block<SEM, N1LOG>(p, n1_end, blockIdx.x,
+                        partials + (size_t)blockIdx.x * kSlots, n1log);
+}
+
+// Per-band log reduction: out[n1] = sum over the unoccupied-kernel blocks
+// (records [rec][n1 - u_begin]) plus sum over the occupied-kernel blocks
+// (records [rec][n1]), each in record order.  One block per band.
+__global__ __launch_bounds__(256) void reduce_n1_kernel(
+    const double *__restrict__ src_u, int nrec_u, int u_begin, int nb_u,
+    const double *__restrict__ src_o, int nrec_o, int nb_o,
+    double *__restrict__ out) {
+  __shared__ double buf[2][256];
+  const int n1 = blockIdx.x;
+  double ar = 0.0, ai = 0.0;
+  if (n1 >= u_begin && n1 - u_begin < nb_u) {
+    const int j = n1 - u_begin;
+    for (int r = threadIdx.x; r < nrec_u; r += 256) {
+      ar += src_u[2 * ((size_t)r * nb_u + j)];
+      ai += src_u[2 * ((size_t)r * nb_u + j) + 1];
+    }
+  }
+  if (n1 < nb_o) {
+    for (int r = threadIdx.x; r < nrec_o; r += 256) {
+      ar += src_o[2 * ((size_t)r * nb_o + n1)];
+      ai += src_o[2 * ((size_t)r * nb_o + n1) + 1];
+    }
+  }
+  buf[0][threadIdx.x] = ar;
+  buf[1][threadIdx.x] = ai;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      buf[0][threadIdx.x] += buf[0][threadIdx.x + h];
+      buf[1][threadIdx.x] += buf[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * n1] = buf[0][0];
+    out[2 * n1 + 1] = buf[1][0];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -616,6 +753,74 @@ __global__ __launch_bounds__(kAchThreads) void achstemp_bgw_kernel(
   if (threadIdx.x >= 2 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// Batched persistent launch: a work list of (problem, kind, block) items
+// sorted largest-first; every block of the grid takes the next item from a
+// global counter until the list is exhausted (every wave reaches the exit
+// test).  Each item writes its own record, so the sums stay in a fixed order
+// whatever block ran them.
+template <int K, int PREC, int SEM>
+__global__ __launch_bounds__(256) void batch_kernel(
+    const DevProblem *__restrict__ probs, const BatchItem *__restrict__ items,
+    int nitems, double *__restrict__ partials, int *__restrict__ counter,
+    float thresh, unsigned long long *__restrict__ fallback_count) {
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const BatchItem w = items[it];
+    const DevProblem p = probs[w.prob];
+    double *rec = partials + (size_t)w.rec * kSlots;
+    if (w.kind == 0) {
+      if (PREC == 0)
+        unocc_fp64_block<K, SEM == kSemBgw, false>(p, p.NV, p.NB, w.block, rec,
+                                                   nullptr);
+      else
+        unocc_mixed_block<K, false>(p, p.NV, p.NB, w.block, rec, nullptr,
+                                    thresh, fallback_count);
+    } else {
+      occ_block<SEM, false>(p, p.NV, w.block, rec, nullptr);
+    }
+  }
+}
+
+__global__ void batch_reset_kernel(int *counter,
+                                   unsigned long long *fallback_count) {
+  if (threadIdx.x == 0) {
+    *counter = 0;
+    *fallback_count = 0ull;
+  }
+}
+
+// One block per problem: fixed-order sum of its records (as reduce_kernel).
+__global__ __launch_bounds__(kRedThreads) void reduce_batch_kernel(
+    const double *__restrict__ partials, const int *__restrict__ rec_begin,
+    const int *__restrict__ rec_count, double *__restrict__ out) {
+  __shared__ double buf[kRedThreads];
+  const double *part = partials + (size_t)rec_begin[blockIdx.x] * kSlots;
+  const int nrec = rec_count[blockIdx.x];
+  for (int s = 0; s < kSlots; ++s) {
+    double acc = 0.0, c = 0.0;
+    for (int r = threadIdx.x; r < nrec; r += kRedThreads) {
+      const double x = part[(size_t)r * kSlots + s];
+      const double t = acc + x;
+      c += (fabs(acc) >= fabs(x)) ? ((acc - t) + x) : ((x - t) + acc);
+      acc = t;
+    }
+    buf[threadIdx.x] = acc + c;
+    __syncthreads();
+    for (int h = kRedThreads / 2; h > 0; h >>= 1) {
+      if (threadIdx.x < h) buf[threadIdx.x] += buf[threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)blockIdx.x * kSlots + s] = buf[0];
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -654,44 +859,57 @@ template <int K>
 static hipError_t launch_unocc_k(const DevProblem &p, int n1b, int n1e,
                                  int precision, int sem, int waves,
                                  double thresh, double *partials,
-                                 unsigned long long *fb, int grid,
-                                 hipStream_t s) {
+                                 double *n1log, unsigned long long *fb,
+                                 int grid, hipStream_t s) {
   const dim3 blk(kWave * waves);
+  const bool lg = n1log != nullptr;
   if (precision == 0) {
-    if (sem == kSemBgw)
-      unocc_fp64_kernel<K, true><<<grid, blk, 0, s>>>(p, n1b, n1e, partials,
-                                                       0, 0);
-    else
-      unocc_fp64_kernel<K, false><<<grid, blk, 0, s>>>(p, n1b, n1e, partials,
-                                                        0, 0);
+    if (sem == kSemBgw) {
+      if (lg)
+        unocc_fp64_kernel<K, true, true>
+            <<<grid, blk, 0, s>>>(p, n1b, n1e, partials, n1log);
+      else
+        unocc_fp64_kernel<K, true, false>
+            <<<grid, blk, 0, s>>>(p, n1b, n1e, partials, n1log);
+    } else {
+      if (lg)
+        unocc_fp64_kernel<K, false, true>
+            <<<grid, blk, 0, s>>>(p, n1b, n1e, partials, n1log);
+      else
+        unocc_fp64_kernel<K, false, false>
+            <<<grid, blk, 0, s>>>(p, n1b, n1e, partials, n1log);
+    }
   } else {
     if (sem == kSemBgw) return hipErrorNotSupported;
-    unocc_mixed_kernel<K><<<grid, blk, 0, s>>>(p, n1b, n1e, partials, 0, 0,
-                                               (float)thresh, fb);
+    if (lg)
+      unocc_mixed_kernel<K, true><<<grid, blk, 0, s>>>(
+          p, n1b, n1e, partials, n1log, (float)thresh, fb);
+    else
+      unocc_mixed_kernel<K, false><<<grid, blk, 0, s>>>(
+          p, n1b, n1e, partials, n1log, (float)thresh, fb);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
                         int precision, int sem, const TileCfg &t,
-                        double res_thresh, double *partials,
+                        double res_thresh, double *partials, double *n1log,
                         unsigned long long *fallback_count, hipStream_t s) {
   const int grid = unocc_grid(p, t);
-  if (grid == 0) return hipSuccess;
-  if (t.waves < 1 || t.waves > 16) return hipErrorInvalidValue;
+  if (grid == 0 || n1_begin >= n1_end) return hipSuccess;
+  // __launch_bounds__(256): at most 4 waves per block.
+  if (t.waves < 1 || t.waves > 4) return hipErrorInvalidValue;
   switch (t.k_ig) {
-    case 4:
-      return launch_unocc_k<4>(p, n1_begin, n1_end, precision, sem, t.waves,
-                               res_thresh, partials, fallback_count, grid, s);
-    case 8:
-      return launch_unocc_k<8>(p, n1_begin, n1_end, precision, sem, t.waves,
-                               res_thresh, partials, fallback_count, grid, s);
-    case 16:
-      return launch_unocc_k<16>(p, n1_begin, n1_end, precision, sem, t.waves,
-                                res_thresh, partials, fallback_count, grid, s);
-    case 32:
-      return launch_unocc_k<32>(p, n1_begin, n1_end, precision, sem, t.waves,
-                                res_thresh, partials, fallback_count, grid, s);
+#define GPP_CASE(KK)                                                        \
+  case KK:                                                                  \
+    return launch_unocc_k<KK>(p, n1_begin, n1_end, precision, sem, t.waves, \
+                              res_thresh, partials, n1log, fallback_count,  \
+                              grid, s);
+    GPP_CASE(4)
+    GPP_CASE(8)
+    GPP_CASE(16)
+    GPP_CASE(32)
+#undef GPP_CASE
     default:
       return hipErrorInvalidValue;
   }
@@ -702,21 +920,30 @@ int occ_grid(const DevProblem &p) {
   return p.NG * ((p.NC + kOccChunk - 1) / kOccChunk);
 }
 
+template <int SEM>
+static void launch_occ_sem(const DevProblem &p, int n1_end, double *partials,
+                           double *n1log, int grid, hipStream_t s) {
+  if (n1log)
+    occ_kernel<SEM, true><<<grid, kOccThreads, 0, s>>>(p, n1_end, partials,
+                                                       n1log);
+  else
+    occ_kernel<SEM, false><<<grid, kOccThreads, 0, s>>>(p, n1_end, partials,
+                                                        n1log);
+}
+
 hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
-                      double *partials, hipStream_t s) {
+                      double *partials, double *n1log, hipStream_t s) {
   const int grid = occ_grid(p);
-  if (grid == 0) return hipSuccess;
+  if (grid == 0 || n1_end <= 0) return hipSuccess;
   switch (sem) {
     case kSemGppker:
-      occ_kernel<kSemGppker><<<grid, kOccThreads, 0, s>>>(p, n1_end, partials,
-                                                          0);
+      launch_occ_sem<kSemGppker>(p, n1_end, partials, n1log, grid, s);
       break;
     case kSemGppkerStrict:
-      occ_kernel<kSemGppkerStrict><<<grid, kOccThreads, 0, s>>>(
-          p, n1_end, partials, 0);
+      launch_occ_sem<kSemGppkerStrict>(p, n1_end, partials, n1log, grid, s);
       break;
     case kSemBgw:
-      occ_kernel<kSemBgw><<<grid, kOccThreads, 0, s>>>(p, n1_end, partials, 0);
+      launch_occ_sem<kSemBgw>(p, n1_end, partials, n1log, grid, s);
       break;
     default:
       return hipErrorInvalidValue;
@@ -727,6 +954,15 @@ hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
 hipError_t launch_reduce(const double *partials, int nrec, double *out,
                          hipStream_t s) {
   reduce_kernel<<<1, kRedThreads, 0, s>>>(partials, nrec, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,
+                            int nb_u, const double *src_o, int nrec_o,
+                            int nb_o, int NB, double *out, hipStream_t s) {
+  if (NB <= 0) return hipSuccess;
+  reduce_n1_kernel<<<NB, 256, 0, s>>>(src_u, nrec_u, u_begin, nb_u, src_o,
+                                      nrec_o, nb_o, out);
   return hipGetLastError();
 }
 
@@ -746,6 +982,98 @@ hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
   else
     achstemp_gppker_kernel<<<n, kAchThreads, 0, s>>>(p, partials, 0);
   return hipGetLastError();
+}
+
+// ---- batch
+
+template <int K, int PREC, int SEM>
+static const void *batch_fn() {
+  return reinterpret_cast<const void *>(&batch_kernel<K, PREC, SEM>);
+}
+
+static const void *batch_entry(int k, int precision, int sem) {
+#define GPP_B(KK)                                                       \
+  if (k == KK) {                                                        \
+    if (precision == 1) return batch_fn<KK, 1, kSemGppker>();           \
+    if (sem == kSemGppkerStrict) return batch_fn<KK, 0, kSemGppkerStrict>(); \
+    if (sem == kSemBgw) return batch_fn<KK, 0, kSemBgw>();              \
+    return batch_fn<KK, 0, kSemGppker>();                               \
+  }
+  GPP_B(4)
+  GPP_B(8)
+  GPP_B(16)
+  GPP_B(32)
+#undef GPP_B
+  return nullptr;
+}
+
+int batch_blocks_per_cu(int k_ig, int precision, int sem) {
+  const void *f = batch_entry(k_ig, precision, sem);
+  if (!f) return 0;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) !=
+      hipSuccess)
+    return 1;
+  return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_reduce_batch(const double *partials, const int *rec_begin,
+                               const int *rec_count, int nprob, double *out,
+                               hipStream_t s) {
+  if (nprob <= 0) return hipSuccess;
+  reduce_batch_kernel<<<nprob, kRedThreads, 0, s>>>(partials, rec_begin,
+                                                    rec_count, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch(const DevProblem *probs, const BatchItem *items,
+                        int nitems, int k_ig, int precision, int sem,
+                        double res_thresh, double *partials, int *counter,
+                        unsigned long long *fallback_count, int grid,
+                        const int *rec_begin, const int *rec_count, int nprob,
+                        double *out, hipStream_t s) {
+  if (precision == 1 && sem == kSemBgw) return hipErrorNotSupported;
+  batch_reset_kernel<<<1, 64, 0, s>>>(counter, fallback_count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (nitems > 0) {
+#define GPP_BL(KK)                                                           \
+  case KK:                                                                   \
+    if (precision == 1)                                                      \
+      batch_kernel<KK, 1, kSemGppker><<<grid, 256, 0, s>>>(                  \
+          probs, items, nitems, partials, counter, (float)res_thresh,        \
+          fallback_count);                                                   \
+    else if (sem == kSemGppkerStrict)                                        \
+      batch_kernel<KK, 0, kSemGppkerStrict><<<grid, 256, 0, s>>>(            \
+          probs, items, nitems, partials, counter, (float)res_thresh,        \
+          fallback_count);                                                   \
+    else if (sem == kSemBgw)                                                 \
+      batch_kernel<KK, 0, kSemBgw><<<grid, 256, 0, s>>>(                     \
+          probs, items, nitems, partials, counter, (float)res_thresh,        \
+          fallback_count);                                                   \
+    else                                                                     \
+      batch_kernel<KK, 0, kSemGppker><<<grid, 256, 0, s>>>(                  \
+          probs, items, nitems, partials, counter, (float)res_thresh,        \
+          fallback_count);                                                   \
+    break;
+    switch (k_ig) {
+      GPP_BL(4)
+      GPP_BL(8)
+      GPP_BL(16)
+      GPP_BL(32)
+      default:
+        return hipErrorInvalidValue;
+    }
+#undef GPP_BL
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (nprob > 0) {
+    reduce_batch_kernel<<<nprob, kRedThreads, 0, s>>>(partials, rec_begin,
+                                                      rec_count, out);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 }  // namespace gpp

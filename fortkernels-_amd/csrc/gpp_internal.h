@@ -42,6 +42,14 @@ struct TileCfg {
   int waves = 4;          // waves per block (each takes its own k_ig run)
 };
 
+// One work item of the batched persistent launch.
+struct BatchItem {
+  int prob;   // index into the problem table
+  int kind;   // 0 = non-occupied block, 1 = occupied block
+  int block;  // block index within that kernel's grid for the problem
+  int rec;    // global record index of its partial sums
+};
+
 // Launchers (gpp_device.hip).  All enqueue on `s` and return hipError_t.
 hipError_t launch_prepass(const int *inv_igp_index, const int *indinv,
                           const double *wx_in, int wx_per_band, int NB,
@@ -50,17 +58,34 @@ hipError_t launch_prepass(const int *inv_igp_index, const int *indinv,
 hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, size_t n,
                                hipStream_t s);
 int unocc_grid(const DevProblem &p, const TileCfg &t);
+// n1log: NULL, or [grid][n1_end - n1_begin] complex per-band records.
 hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
                         int precision, int sem, const TileCfg &t,
-                        double res_thresh, double *partials,
+                        double res_thresh, double *partials, double *n1log,
                         unsigned long long *fallback_count, hipStream_t s);
 int occ_grid(const DevProblem &p);
+// n1log: NULL, or [grid][n1_end] complex per-band records.
 hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
-                      double *partials, hipStream_t s);
+                      double *partials, double *n1log, hipStream_t s);
 hipError_t launch_reduce(const double *partials, int nrec, double *out,
                          hipStream_t s);
+hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,
+                            int nb_u, const double *src_o, int nrec_o,
+                            int nb_o, int NB, double *out, hipStream_t s);
 int achstemp_records(const DevProblem &p, int sem);
 hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
                            hipStream_t s);
+// Batched persistent launch (block size 256 = 4 waves for the non-occupied
+// items) followed by one reduction block per problem into out[prob][12].
+int batch_blocks_per_cu(int k_ig, int precision, int sem);
+hipError_t launch_reduce_batch(const double *partials, const int *rec_begin,
+                               const int *rec_count, int nprob, double *out,
+                               hipStream_t s);
+hipError_t launch_batch(const DevProblem *probs, const BatchItem *items,
+                        int nitems, int k_ig, int precision, int sem,
+                        double res_thresh, double *partials, int *counter,
+                        unsigned long long *fallback_count, int grid,
+                        const int *rec_begin, const int *rec_count, int nprob,
+                        double *out, hipStream_t s);
 
 }  // namespace gpp

@@ -75,6 +75,8 @@ typedef struct { double re, im; } gpp_c128;
                                         gppKer_gpuComplex.cpp:127-145)     */
 #define GPP_FLAG_CHECK_FINITE     4  /* return GPP_ENONFINITE on NaN/Inf   */
 #define GPP_FLAG_NO_ACHSTEMP      8  /* skip the static remainder achstemp */
+#define GPP_FLAG_AUTOTUNE        16  /* tune the tile shape for this shape
+                                        class first if it is not cached    */
 
 /* One sigma evaluation.  Shapes and arrays as in gppKer.cpp:157-264. */
 typedef struct {
@@ -104,7 +106,7 @@ typedef struct {
   void *stream;                   /* hipStream_t for GPP_MEM_DEVICE work;
                                      NULL = the handle's own stream        */
   double resonance_threshold;     /* mixed: FP64 for |wdiff|^2 below this;
-                                     <= 0 selects the default 1e-2         */
+                                     <= 0 selects the default 1e-6         */
   int tile_ig;                    /* G-vectors per lane run (0 = tuned)    */
   int waves_per_block;            /* 0 = tuned                             */
 } gpp_options;
@@ -137,10 +139,21 @@ void gpp_default_options(gpp_options *o);
 int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
                  gpp_result *r);
 
-/* Many independent evaluations in one persistent launch (FP64 or mixed).
- * All problems must live in the same memory space (o->memory). */
+/* Many independent evaluations in one persistent launch (FP64 or mixed),
+ * captured in a hipGraph that is replayed while the batch (pointers, shapes,
+ * options) is unchanged.  All problems use the same options and memory
+ * space; band-range flags and acht_n1_loc are not available here. */
 int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
                     const gpp_options *o, gpp_result *r);
+
+/* Tile autotuner: times the non-occupied kernel on a slice of at most 32
+ * unoccupied bands of `p` for every candidate (G-vectors per lane run x
+ * waves per block), keeps the fastest for the problem's shape class
+ * (precision, semantics, log2 ncouls, log2 ngpown) in the handle, appends it
+ * to the file named by $GPP_TUNE_CACHE (read back by gpp_create), and
+ * returns it.  Later calls with tile_ig = waves_per_block = 0 use it. */
+int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *o,
+                 int *tile_ig, int *waves_per_block);
 
 /*
  * Drop-in for the band loop of gppKer.cpp:270-361.  Host pointers,

@@ -56,7 +56,7 @@ def test_random_fp64_vs_oracle(handle, shape, sem):
     assert abs(got["achstemp"] - ref["achstemp"]) <= 1e-11 * abs(ref["achstemp"])
 
 
-@pytest.mark.parametrize("tile", [(4, 1), (8, 2), (8, 4), (16, 4), (32, 2), (16, 16)])
+@pytest.mark.parametrize("tile", [(4, 1), (8, 2), (8, 4), (16, 4), (32, 2), (16, 1)])
 def test_tile_shapes_agree(handle, tile):
     k, w = tile
     say("tile", tile)
@@ -205,3 +205,82 @@ def test_canonical_shape_band_subset(handle):
     parts = [handle.sigma(gppk.band_slice(inp, a, b)) for a, b in ((0, 100), (100, 512))]
     for k in ("achtemp", "asxtemp"):
         assert rel_err(parts[0][k] + parts[1][k], full[k]) <= 1e-12
+
+
+def test_waves_over_launch_bound_rejected_on_host(handle):
+    inp = gppk.fill_inputs(2, 2, 256, 4, nvband=0, seed=5)
+    with pytest.raises(gppk.GppError) as ei:
+        handle.sigma(inp, waves_per_block=8)
+    assert ei.value.code == gppk.GPP_EINVAL
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_acht_n1_loc(handle, sem):
+    say("acht_n1_loc", sem)
+    inp = gppk.fill_inputs(3, 10, 700, 7, nvband=3, seed=21)
+    got = handle.sigma(inp, semantics=sem, acht_n1_loc=True)
+    ref = orc.sigma(inp, sem, want_acht_n1=True)
+    check(got, ref)
+    e = rel_err(got["acht_n1_loc"], ref["acht_n1_loc"])
+    assert e <= FP64_TOL, e
+    # the per-band log sums to achtemp[2]
+    assert rel_err(np.sum(got["acht_n1_loc"]), got["achtemp"][2]) <= 1e-12
+
+
+def test_acht_n1_loc_mixed(handle):
+    inp = gppk.fill_inputs(2, 6, 512, 8, nvband=2, seed=22)
+    got = handle.sigma(inp, precision="mixed", acht_n1_loc=True)
+    ref = orc.sigma(inp, want_acht_n1=True)
+    assert rel_err(got["acht_n1_loc"], ref["acht_n1_loc"]) < 2e-6
+
+
+def _batch_inputs(seed=31, n=7):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nb = int(rng.choice([3, 5, 8, 12]))
+        nc = int(rng.integers(64, 900))
+        nv = int(rng.integers(0, 4))
+        cfg = 3 if i % 2 else 2
+        out.append(gppk.fill_inputs(cfg, nb, nc, 4, nvband=nv, seed=100 + i))
+    return out
+
+
+@pytest.mark.parametrize("sem", ["gppker", "strict", "bgw"])
+def test_batch_fp64_vs_oracle_and_graph_replays(handle, sem):
+    say("batch", sem)
+    inputs = _batch_inputs()
+    refs = [orc.sigma(d, sem) for d in inputs]
+    for rep in range(3):
+        res = handle.batch(inputs, semantics=sem)
+        for got, ref in zip(res, refs):
+            check(got, ref)
+            assert abs(got["achstemp"] - ref["achstemp"]) <= 1e-11 * abs(ref["achstemp"])
+        # eager work on the same handle between graph replays
+        check(handle.sigma(inputs[0], semantics=sem), refs[0])
+
+
+def test_batch_mixed(handle):
+    say("batch mixed")
+    inputs = _batch_inputs(seed=32, n=5)
+    res = handle.batch(inputs, precision="mixed")
+    for got, d in zip(res, inputs):
+        ref = orc.sigma(d)
+        assert rel_err(got["achtemp"], ref["achtemp"]) < 2e-6
+        assert rel_err(got["asxtemp"], ref["asxtemp"]) <= FP64_TOL
+
+
+def test_batch_rejects_unsupported(handle):
+    inputs = _batch_inputs(n=2)
+    with pytest.raises(gppk.GppError):
+        handle.batch(inputs, flags=gppk.FLAG_ALL_BANDS_UNOCC)
+
+
+def test_autotune_picks_valid_tile_and_agrees(handle):
+    say("autotune")
+    inp = gppk.fill_inputs(2, 40, 2048, 8, nvband=2, seed=41)
+    k, w = handle.autotune(inp)
+    assert k in (4, 8, 16) and w in (1, 2, 4)
+    ref = orc.sigma(inp)
+    check(handle.sigma(inp), ref)            # uses the tuned tile
+    check(handle.sigma(inp, flags=gppk.FLAG_AUTOTUNE), ref)
