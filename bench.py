@@ -108,7 +108,27 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--precisions", default="fp64,mixed")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE/"
+                         "WRITE_SIZE to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = (1 << 30) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 5
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
+                          "ms": dt * 1e3, "GB_per_s": 2 * n * 8 / dt / 1e9}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the sum would shard over G' columns with one all-reduce of 12 "

@@ -263,21 +263,33 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
       // Some product d0*d1*d2 was zero, denormal, infinite or NaN: redo this
       // lane's run with one division per term so that exact resonances give
       // the reference's inf/NaN in their own frequency only.
+      // The operands are reloaded (this path is rare) and the loop is kept
+      // rolled, so that it adds no register pressure to the hot loop.
       s_r0 = s_i0 = s_r1 = s_i1 = s_r2 = s_i2 = 0.0;
+#pragma unroll 1
       for (int k = 0; k < K; ++k) {
-        const int igk = min(ig0 + k, NC - 1);
-        const double2 an = an_row[igk];
-        const double Br = an.x * Ar[k] - an.y * Ai[k];
-        const double Bi = an.x * Ai[k] + an.y * Ar[k];
+        const int ig = ig0 + k;
+        if (!valid || ig >= NC) continue;  // inert slot: contributes 0
+        const size_t off = (size_t)my_igp * NC + ig;
+        const double2 w = p.wtilde[off];
+        const double2 e = p.ieps[off];
+        const double A_r = w.x * e.x - w.y * e.y;
+        const double A_i = w.x * e.y + w.y * e.x;
+        const double2 an = an_row[ig];
+        const double Br = an.x * A_r - an.y * A_i;
+        const double Bi = an.x * A_i + an.y * A_r;
+        const double wy2 = w.y * w.y;
         const double xs[3] = {x0, x1, x2};
         double sr[3], si[3];
+#pragma unroll
         for (int iw = 0; iw < 3; ++iw) {
-          const double dr = xs[iw] - wr[k];
-          const double d = fma(dr, dr, wi2[k]);
+          const double dr = xs[iw] - w.x;
+          const double d = fma(dr, dr, wy2);
           double r = 1.0 / d;
-          if (MASK) r = (d > kLimitTwo && wt2[k] * r < kLimitOne) ? r : 0.0;
-          sr[iw] = (Br * dr - Bi * wi[k]) * r;
-          si[iw] = (Br * wi[k] + Bi * dr) * r;
+          if (MASK)
+            r = (d > kLimitTwo && (w.x * w.x + wy2) * r < kLimitOne) ? r : 0.0;
+          sr[iw] = (Br * dr - Bi * w.y) * r;
+          si[iw] = (Br * w.y + Bi * dr) * r;
         }
         s_r0 += sr[0];
         s_i0 += si[0];
@@ -408,18 +420,24 @@ __device__ __forceinline__ void unocc_mixed_block(
       }
     }
     if (__builtin_expect(bad, 0)) {
-      // The same FP32 test picks exactly the terms the loop above left out.
-#pragma unroll
+      // The same FP32 test, on operands rebuilt exactly as the tile load
+      // built them, picks exactly the terms the loop above left out.  Rolled
+      // loop with reloads: no register pressure added to the hot loop.
+#pragma unroll 1
       for (int k = 0; k < K; ++k) {
+        const int ig = ig0 + k;
+        if (!valid || ig >= NC) continue;  // inert slot: never excluded
+        const size_t off = (size_t)my_igp * NC + ig;
+        const double2 w = p.wtilde[off];
+        const float wif = (float)w.y;
+        const float wi2f = wif * wif;
 #pragma unroll
         for (int iw = 0; iw < 3; ++iw) {
-          const float dr = (float)(x[iw] - wr[k]);
-          const float d = fmaf(dr, dr, wi2[k]);
-          if (!(d >= thresh) && valid && ig0 + k < NC) {
-            const size_t off = (size_t)my_igp * NC + (ig0 + k);
-            const double2 w = p.wtilde[off];
+          const float dr = (float)(x[iw] - w.x);
+          const float d = fmaf(dr, dr, wi2f);
+          if (!(d >= thresh)) {
             const double2 e = p.ieps[off];
-            const double2 a = p.aqsn[(size_t)n1 * NC + (ig0 + k)];
+            const double2 a = p.aqsn[(size_t)n1 * NC + ig];
             const double A_r = w.x * e.x - w.y * e.y;
             const double A_i = w.x * e.y + w.y * e.x;
             const double B_r = a.x * A_r - a.y * A_i;
