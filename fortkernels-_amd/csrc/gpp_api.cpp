@@ -186,7 +186,8 @@ TuneKey tune_key(const gpp_problem *p, const gpp_options *o) {
 
 TileCfg default_tile(int precision) {
   TileCfg t;
-  t.k_ig = precision == GPP_PREC_MIXED ? 16 : 8;
+  // Fastest untuned tiles at the canonical 512/2/32768/20 shape on MI355X.
+  t.k_ig = precision == GPP_PREC_MIXED ? 8 : 16;
   t.waves = 4;
   return t;
 }
@@ -277,12 +278,14 @@ int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
     if ((rc = ensure(h, h->stage, sa.used))) return rc;
   }
   Arena wa;
-  std::vector<size_t> woff((size_t)n * 3);
+  std::vector<size_t> woff((size_t)n * 4);
   for (int i = 0; i < n; ++i) {
     const size_t nbnc = (size_t)p[i].number_bands * p[i].ncouls;
-    woff[(size_t)i * 3 + 0] = wa.take((size_t)p[i].ngpown * sizeof(int));
-    woff[(size_t)i * 3 + 1] = wa.take((size_t)p[i].number_bands * 3 * 8);
-    woff[(size_t)i * 3 + 2] = mixed ? wa.take(nbnc * sizeof(float2)) : 0;
+    const size_t nbng = (size_t)p[i].number_bands * p[i].ngpown;
+    woff[(size_t)i * 4 + 0] = wa.take((size_t)p[i].ngpown * sizeof(int));
+    woff[(size_t)i * 4 + 1] = wa.take((size_t)p[i].number_bands * 3 * 8);
+    woff[(size_t)i * 4 + 2] = mixed ? wa.take(nbnc * sizeof(float2)) : 0;
+    woff[(size_t)i * 4 + 3] = wa.take(nbng * 16);
   }
   if ((rc = ensure(h, h->work, wa.used))) return rc;
   char *sbase = (char *)h->stage.ptr;
@@ -314,10 +317,11 @@ int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
     q.indinv = (const int *)dev[6];
     q.wx_in = (const double *)dev[7];
     q.wx_per_band = p[i].wx_per_band;
-    q.dp.igp_of = (const int *)(wbase + woff[(size_t)i * 3 + 0]);
-    q.dp.wx = (const double *)(wbase + woff[(size_t)i * 3 + 1]);
+    q.dp.igp_of = (const int *)(wbase + woff[(size_t)i * 4 + 0]);
+    q.dp.wx = (const double *)(wbase + woff[(size_t)i * 4 + 1]);
     q.dp.aqsn32 =
-        mixed ? (const float2 *)(wbase + woff[(size_t)i * 3 + 2]) : nullptr;
+        mixed ? (const float2 *)(wbase + woff[(size_t)i * 4 + 2]) : nullptr;
+    q.dp.aqsmG = (const double2 *)(wbase + woff[(size_t)i * 4 + 3]);
     q.dp.NB = p[i].number_bands;
     q.dp.NV = std::min(p[i].nvband, p[i].number_bands);
     q.dp.NC = p[i].ncouls;
@@ -339,6 +343,9 @@ int enqueue_prepass(gpp_handle h, const Prepared &q, bool mixed,
                                     q.dp.NB, q.dp.NC, q.dp.NG,
                                     (int *)q.dp.igp_of, (double *)q.dp.wx,
                                     counter, s));
+  HIP_OR_RET(h, gpp::launch_gather_aqsm(q.dp.aqsm, q.dp.igp_of, q.dp.NB,
+                                        q.dp.NC, q.dp.NG,
+                                        (double2 *)q.dp.aqsmG, s));
   if (mixed && q.dp.NB > q.dp.NV)
     HIP_OR_RET(h, gpp::launch_aqsn_to_fp32(
                       q.dp.aqsn, (float2 *)q.dp.aqsn32,

@@ -135,6 +135,24 @@ __global__ void prepass_kernel(const int *__restrict__ inv_igp_index,
   }
 }
 
+// aqsmG[n1][my_igp] = aqsm[n1][igp_of[my_igp]] (0 for skipped rows): the
+// per-lane aqsm operand of the non-occupied kernels becomes one coalesced
+// row per band instead of a gather with stride nodes_per_group.
+__global__ void gather_aqsm_kernel(const double2 *__restrict__ aqsm,
+                                   const int *__restrict__ igp_of, int NB,
+                                   int NC, int NG,
+                                   double2 *__restrict__ out) {
+  const size_t n = (size_t)NB * NG;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const int n1 = (int)(i / NG);
+    const int g = (int)(i % NG);
+    const int igp = igp_of[g];
+    out[i] = igp >= 0 ? aqsm[(size_t)n1 * NC + igp] : make_double2(0.0, 0.0);
+  }
+}
+
 __global__ void to_fp32_kernel(const double2 *__restrict__ in,
                                float2 *__restrict__ out, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -175,7 +193,6 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
 
   int igp = (my_igp < NG) ? p.igp_of[my_igp] : -1;
   const bool valid = igp >= 0;
-  const int igp_c = valid ? igp : 0;
 
   double wr[K], wi[K], wi2[K], Ar[K], Ai[K];
   double wt2[K];  // |wtilde|^2, only for the mask
@@ -204,14 +221,15 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
   // sch_array[iw] = 0.5 * sum; achtemp[iw] += sch_array[iw] * vcoul[igp]
   const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double acc_r[3] = {0.0, 0.0, 0.0}, acc_i[3] = {0.0, 0.0, 0.0};
-  const double2 *aqsm_col = p.aqsm + igp_c;
-  double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NC]
+  // aqsm gathered per band into [n1][my_igp] (coalesced across lanes).
+  const double2 *aqsm_col = p.aqsmG + min(my_igp, NG - 1);
+  double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NG]
                                         : make_double2(0.0, 0.0);
 
   for (int n1 = n1_begin; n1 < n1_end; ++n1) {
     const double2 am = am_next;
     // Prefetch next band's aqsm (index clamped: no branch in the loop).
-    am_next = aqsm_col[(size_t)min(n1 + 1, n1_end - 1) * NC];
+    am_next = aqsm_col[(size_t)min(n1 + 1, n1_end - 1) * NG];
     const double x0 = p.wx[n1 * 3 + 0];
     const double x1 = p.wx[n1 * 3 + 1];
     const double x2 = p.wx[n1 * 3 + 2];
@@ -356,7 +374,6 @@ __device__ __forceinline__ void unocc_mixed_block(
 
   int igp = (my_igp < NG) ? p.igp_of[my_igp] : -1;
   const bool valid = igp >= 0;
-  const int igp_c = valid ? igp : 0;
 
   double wr[K];
   float wi[K], wi2[K], Ar[K], Ai[K];
@@ -385,13 +402,14 @@ __device__ __forceinline__ void unocc_mixed_block(
   const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double acc_r[3] = {0.0, 0.0, 0.0}, acc_i[3] = {0.0, 0.0, 0.0};
   unsigned long long nfb = 0;
-  const double2 *aqsm_col = p.aqsm + igp_c;
-  double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NC]
+  // aqsm gathered per band into [n1][my_igp] (coalesced across lanes).
+  const double2 *aqsm_col = p.aqsmG + min(my_igp, NG - 1);
+  double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NG]
                                         : make_double2(0.0, 0.0);
 
   for (int n1 = n1_begin; n1 < n1_end; ++n1) {
     const double2 am = am_next;
-    am_next = aqsm_col[(size_t)min(n1 + 1, n1_end - 1) * NC];
+    am_next = aqsm_col[(size_t)min(n1 + 1, n1_end - 1) * NG];
     const double x[3] = {p.wx[n1 * 3 + 0], p.wx[n1 * 3 + 1],
                          p.wx[n1 * 3 + 2]};
     const float2 *an_row = p.aqsn32 + (size_t)n1 * NC;
@@ -675,57 +693,54 @@ __global__ __launch_bounds__(256) void reduce_n1_kernel(
 
 // ---------------------------------------------------------------------------
 // Fixed-order sum of `nrec` records of kSlots doubles -> out[kSlots].
-// One block: thread t sums records t, t+256, ... of one slot group in
-// order; then a fixed pairwise tree over the 256 partials.
+// One block per slot: thread t sums records t, t+256, ... in order
+// (Neumaier-compensated), then a fixed pairwise tree over the 256 partials.
 constexpr int kRedThreads = 256;
 __global__ __launch_bounds__(kRedThreads) void reduce_kernel(
     const double *__restrict__ partials, int nrec, double *__restrict__ out) {
   __shared__ double buf[kRedThreads];
-  for (int s = 0; s < kSlots; ++s) {
-    double acc = 0.0, c = 0.0;  // Neumaier compensation
-    for (int r = threadIdx.x; r < nrec; r += kRedThreads) {
-      const double x = partials[(size_t)r * kSlots + s];
-      const double t = acc + x;
-      c += (fabs(acc) >= fabs(x)) ? ((acc - t) + x) : ((x - t) + acc);
-      acc = t;
-    }
-    buf[threadIdx.x] = acc + c;
-    __syncthreads();
-    for (int h = kRedThreads / 2; h > 0; h >>= 1) {
-      if (threadIdx.x < h) buf[threadIdx.x] += buf[threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[s] = buf[0];
+  const int s = blockIdx.x;
+  double acc = 0.0, c = 0.0;
+  for (int r = threadIdx.x; r < nrec; r += kRedThreads) {
+    const double x = partials[(size_t)r * kSlots + s];
+    const double t = acc + x;
+    c += (fabs(acc) >= fabs(x)) ? ((acc - t) + x) : ((x - t) + acc);
+    acc = t;
+  }
+  buf[threadIdx.x] = acc + c;
+  __syncthreads();
+  for (int h = kRedThreads / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) buf[threadIdx.x] += buf[threadIdx.x + h];
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[s] = buf[0];
 }
 
 // ---------------------------------------------------------------------------
 // Static remainder achstemp, gppker form (gppKer.cpp:64-120 with the
 // shared-accumulator race removed): per band and my_igp, the igp term
 //   0.5 * vcoul[igp] * aqsn[n1][igp] * conj(aqsm[n1][igp]) * I_eps[my_igp][igp]
-// taken when |I_eps[my_igp][igp]| > to1 (:96-97).  Record slot 0/1.
+// taken when |I_eps[my_igp][igp]| > to1 (:96-97).  One block per my_igp,
+// threads over bands; record slot 0/1.
 constexpr int kAchThreads = 256;
 __global__ __launch_bounds__(kAchThreads) void achstemp_gppker_kernel(
     DevProblem p, double *__restrict__ partials, int rec_base) {
-  const int my_igp = blockIdx.x * kAchThreads + threadIdx.x;
+  const int my_igp = blockIdx.x;
+  const int igp = p.igp_of[my_igp];
   double vr = 0.0, vi = 0.0;
-  if (my_igp < p.NG) {
-    const int igp = p.igp_of[my_igp];
-    if (igp >= 0) {
-      const double2 s = p.ieps[(size_t)my_igp * p.NC + igp];
-      if (sqrt(s.x * s.x + s.y * s.y) > kTo1) {
-        double tr = 0.0, ti = 0.0;
-        for (int n1 = 0; n1 < p.NB; ++n1) {
-          const double2 am = p.aqsm[(size_t)n1 * p.NC + igp];
-          const double2 an = p.aqsn[(size_t)n1 * p.NC + igp];
-          tr += an.x * am.x + an.y * am.y;
-          ti += an.y * am.x - an.x * am.y;
-        }
-        const double w = 0.5 * p.vcoul[igp];
-        vr = w * (tr * s.x - ti * s.y);
-        vi = w * (tr * s.y + ti * s.x);
+  if (igp >= 0) {  // block-uniform
+    const double2 s = p.ieps[(size_t)my_igp * p.NC + igp];
+    if (sqrt(s.x * s.x + s.y * s.y) > kTo1) {
+      double tr = 0.0, ti = 0.0;
+      for (int n1 = threadIdx.x; n1 < p.NB; n1 += kAchThreads) {
+        const double2 am = p.aqsm[(size_t)n1 * p.NC + igp];
+        const double2 an = p.aqsn[(size_t)n1 * p.NC + igp];
+        tr += an.x * am.x + an.y * am.y;
+        ti += an.y * am.x - an.x * am.y;
       }
+      const double w = 0.5 * p.vcoul[igp];
+      vr = w * (tr * s.x - ti * s.y);
+      vi = w * (tr * s.y + ti * s.x);
     }
   }
   double v[2] = {vr, vi};
@@ -856,6 +871,17 @@ hipError_t launch_prepass(const int *inv_igp_index, const int *indinv,
   return hipGetLastError();
 }
 
+hipError_t launch_gather_aqsm(const double2 *aqsm, const int *igp_of, int NB,
+                              int NC, int NG, double2 *out, hipStream_t s) {
+  const size_t n = (size_t)NB * NG;
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  gather_aqsm_kernel<<<(unsigned)blocks, 256, 0, s>>>(aqsm, igp_of, NB, NC, NG,
+                                                      out);
+  return hipGetLastError();
+}
+
 hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, size_t n,
                                hipStream_t s) {
   if (n == 0) return hipSuccess;
@@ -971,7 +997,7 @@ hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
 
 hipError_t launch_reduce(const double *partials, int nrec, double *out,
                          hipStream_t s) {
-  reduce_kernel<<<1, kRedThreads, 0, s>>>(partials, nrec, out);
+  reduce_kernel<<<kSlots, kRedThreads, 0, s>>>(partials, nrec, out);
   return hipGetLastError();
 }
 
@@ -988,7 +1014,7 @@ int achstemp_records(const DevProblem &p, int sem) {
   if (p.NG <= 0 || p.NC <= 0 || p.NB <= 0) return 0;
   if (sem == kSemBgw)
     return p.NG * ((p.NB + kAchBands - 1) / kAchBands);
-  return (p.NG + kAchThreads - 1) / kAchThreads;
+  return p.NG;
 }
 
 hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,

@@ -33,6 +33,7 @@ struct DevProblem {
   const int *igp_of;      // [NG]: resolved igp in [0, NC) or -1 (skipped)
   const double *wx;       // [NB][3], clamped, band-resolved
   const float2 *aqsn32;   // [NB][NC] FP32 copy (mixed precision only)
+  const double2 *aqsmG;   // [NB][NG]: aqsm[n1][igp_of[my_igp]], 0 if skipped
   int NB, NV, NC, NG;
 };
 
@@ -57,6 +58,8 @@ hipError_t launch_prepass(const int *inv_igp_index, const int *indinv,
                           unsigned long long *counter, hipStream_t s);
 hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, size_t n,
                                hipStream_t s);
+hipError_t launch_gather_aqsm(const double2 *aqsm, const int *igp_of, int NB,
+                              int NC, int NG, double2 *out, hipStream_t s);
 int unocc_grid(const DevProblem &p, const TileCfg &t);
 // n1log: NULL, or [grid][n1_end - n1_begin] complex per-band records.
 hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
