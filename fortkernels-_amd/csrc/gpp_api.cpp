@@ -187,7 +187,7 @@ TuneKey tune_key(const gpp_problem *p, const gpp_options *o) {
 TileCfg default_tile(int precision) {
   TileCfg t;
   // Fastest untuned tiles at the canonical 512/2/32768/20 shape on MI355X.
-  t.k_ig = precision == GPP_PREC_MIXED ? 8 : 16;
+  t.k_ig = precision == GPP_PREC_MIXED ? 4 : 8;
   t.waves = 4;
   return t;
 }
