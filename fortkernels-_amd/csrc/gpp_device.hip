@@ -351,12 +351,15 @@ __global__ __launch_bounds__(256) void unocc_fp64_kernel(
 
 // ---------------------------------------------------------------------------
 // Non-occupied bands, mixed precision.
-// Same decomposition.  B, Y, |wdiff|^2 and the reciprocal are FP32; the
-// difference dr = wx - wr is taken in FP64 and then rounded, so the
-// catastrophic cancellation near resonance stays in FP64.  Terms with
-// |wdiff|^2 < thresh are re-evaluated fully in FP64 (from the FP64 arrays)
-// and accumulated in FP64.  The per-band run sum S is FP32 over K terms;
-// everything from conj(aqsm)*S on is FP64.
+// Same decomposition as the FP64 kernel; the per-term work is FP32:
+//   dr = (xh - wrh) + (xl - wrl)      wx and wtilde.re split into FP32
+//                                     hi/lo pairs, so the cancellation
+//                                     near resonance is exact
+//   d  = dr^2 + wi^2, r = v_rcp_f32(d), B = aqsn * A, S += B*conj(wdiff)*r
+// with the FP32 run sum S over K G-vectors; conj(aqsm)*S and all later sums
+// are FP64.  If any term of a lane's run has d < thresh (near or exact
+// resonance, FP32 underflow), that lane's whole run for this band is redone
+// in FP64 from the FP64 arrays and replaces the FP32 sum.
 template <int K, bool N1LOG>
 __device__ __forceinline__ void unocc_mixed_block(
     const DevProblem &p, int n1_begin, int n1_end, int b,
@@ -375,8 +378,7 @@ __device__ __forceinline__ void unocc_mixed_block(
   int igp = (my_igp < NG) ? p.igp_of[my_igp] : -1;
   const bool valid = igp >= 0;
 
-  double wr[K];
-  float wi[K], wi2[K], Ar[K], Ai[K];
+  float wrh[K], wrl[K], wi[K], wi2[K], Ar[K], Ai[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int ig = ig0 + k;
@@ -384,14 +386,16 @@ __device__ __forceinline__ void unocc_mixed_block(
       const size_t off = (size_t)my_igp * NC + ig;
       const double2 w = p.wtilde[off];
       const double2 e = p.ieps[off];
-      wr[k] = w.x;
+      wrh[k] = (float)w.x;
+      wrl[k] = (float)(w.x - (double)wrh[k]);
       wi[k] = (float)w.y;
       Ar[k] = (float)(w.x * e.x - w.y * e.y);
       Ai[k] = (float)(w.x * e.y + w.y * e.x);
     } else {
-      // Inert slot: A = 0 and a far-away wtilde, so |wdiff|^2 overflows to
-      // +inf in FP32 (contribution 0, never routed to the FP64 redo).
-      wr[k] = 1e30;
+      // Inert slot: A = 0 and a far-away wtilde: |wdiff|^2 overflows to
+      // +inf, r = 0, contribution 0, never below the threshold.
+      wrh[k] = 1e30f;
+      wrl[k] = 0.0f;
       wi[k] = 0.0f;
       Ar[k] = 0.0f;
       Ai[k] = 0.0f;
@@ -402,7 +406,6 @@ __device__ __forceinline__ void unocc_mixed_block(
   const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double acc_r[3] = {0.0, 0.0, 0.0}, acc_i[3] = {0.0, 0.0, 0.0};
   unsigned long long nfb = 0;
-  // aqsm gathered per band into [n1][my_igp] (coalesced across lanes).
   const double2 *aqsm_col = p.aqsmG + min(my_igp, NG - 1);
   double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NG]
                                         : make_double2(0.0, 0.0);
@@ -412,11 +415,16 @@ __device__ __forceinline__ void unocc_mixed_block(
     am_next = aqsm_col[(size_t)min(n1 + 1, n1_end - 1) * NG];
     const double x[3] = {p.wx[n1 * 3 + 0], p.wx[n1 * 3 + 1],
                          p.wx[n1 * 3 + 2]};
+    float xh[3], xl[3];
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      xh[iw] = (float)x[iw];
+      xl[iw] = (float)(x[iw] - (double)xh[iw]);
+    }
     const float2 *an_row = p.aqsn32 + (size_t)n1 * NC;
 
     float s_r[3] = {0.f, 0.f, 0.f}, s_i[3] = {0.f, 0.f, 0.f};
-    double f_r[3] = {0.0, 0.0, 0.0}, f_i[3] = {0.0, 0.0, 0.0};
-    bool bad = false;
+    float dmin = 3.0e38f;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int igk = min(ig0 + k, NC - 1);
@@ -425,61 +433,59 @@ __device__ __forceinline__ void unocc_mixed_block(
       const float Bi = an.x * Ai[k] + an.y * Ar[k];
       const float nBiwi = -Bi * wi[k];
       const float Brwi = Br * wi[k];
+      float d[3];
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) {
-        const float dr = (float)(x[iw] - wr[k]);
-        const float d = fmaf(dr, dr, wi2[k]);
-        // Near resonance (or NaN): excluded here, redone in FP64 below.
-        const bool nr = !(d >= thresh);
-        bad |= nr;
-        const float r = nr ? 0.0f : rcp_f32(d);
+        const float dr = (xh[iw] - wrh[k]) + (xl[iw] - wrl[k]);
+        d[iw] = fmaf(dr, dr, wi2[k]);
+        const float r = rcp_f32(d[iw]);
         s_r[iw] = fmaf(fmaf(Br, dr, nBiwi), r, s_r[iw]);
         s_i[iw] = fmaf(fmaf(Bi, dr, Brwi), r, s_i[iw]);
       }
+      dmin = fminf(dmin, fminf(d[0], fminf(d[1], d[2])));
     }
-    if (__builtin_expect(bad, 0)) {
-      // The same FP32 test, on operands rebuilt exactly as the tile load
-      // built them, picks exactly the terms the loop above left out.  Rolled
-      // loop with reloads: no register pressure added to the hot loop.
+    double sr[3], si[3];
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      sr[iw] = (double)s_r[iw];
+      si[iw] = (double)s_i[iw];
+    }
+    if (__builtin_expect(!(dmin >= thresh), 0)) {
+      // Rare: the lane's run is redone in FP64 (rolled loop, reloads, so
+      // that it adds no register pressure to the loop above).
+      sr[0] = sr[1] = sr[2] = si[0] = si[1] = si[2] = 0.0;
+      const double2 *an64 = p.aqsn + (size_t)n1 * NC;
 #pragma unroll 1
       for (int k = 0; k < K; ++k) {
         const int ig = ig0 + k;
-        if (!valid || ig >= NC) continue;  // inert slot: never excluded
+        if (!valid || ig >= NC) continue;
         const size_t off = (size_t)my_igp * NC + ig;
         const double2 w = p.wtilde[off];
-        const float wif = (float)w.y;
-        const float wi2f = wif * wif;
+        const double2 e = p.ieps[off];
+        const double2 a = an64[ig];
+        const double A_r = w.x * e.x - w.y * e.y;
+        const double A_i = w.x * e.y + w.y * e.x;
+        const double B_r = a.x * A_r - a.y * A_i;
+        const double B_i = a.x * A_i + a.y * A_r;
 #pragma unroll
         for (int iw = 0; iw < 3; ++iw) {
-          const float dr = (float)(x[iw] - w.x);
-          const float d = fmaf(dr, dr, wi2f);
-          if (!(d >= thresh)) {
-            const double2 e = p.ieps[off];
-            const double2 a = p.aqsn[(size_t)n1 * NC + ig];
-            const double A_r = w.x * e.x - w.y * e.y;
-            const double A_i = w.x * e.y + w.y * e.x;
-            const double B_r = a.x * A_r - a.y * A_i;
-            const double B_i = a.x * A_i + a.y * A_r;
-            const double ddr = x[iw] - w.x;
-            const double dd = ddr * ddr + w.y * w.y;
-            const double rr = 1.0 / dd;
-            f_r[iw] = fma((B_r * ddr - B_i * w.y), rr, f_r[iw]);
-            f_i[iw] = fma((B_r * w.y + B_i * ddr), rr, f_i[iw]);
-            ++nfb;
-          }
+          const double ddr = x[iw] - w.x;
+          const double dd = ddr * ddr + w.y * w.y;
+          const double rr = 1.0 / dd;
+          sr[iw] += (B_r * ddr - B_i * w.y) * rr;
+          si[iw] += (B_r * w.y + B_i * ddr) * rr;
         }
       }
+      nfb += 3 * K;
     }
     double lr = 0.0, li = 0.0;
 #pragma unroll
     for (int iw = 0; iw < 3; ++iw) {
-      const double sr = (double)s_r[iw] + f_r[iw];
-      const double si = (double)s_i[iw] + f_i[iw];
-      acc_r[iw] = fma(am.x, sr, fma(am.y, si, acc_r[iw]));
-      acc_i[iw] = fma(am.x, si, fma(-am.y, sr, acc_i[iw]));
+      acc_r[iw] = fma(am.x, sr[iw], fma(am.y, si[iw], acc_r[iw]));
+      acc_i[iw] = fma(am.x, si[iw], fma(-am.y, sr[iw], acc_i[iw]));
       if (iw == 2) {
-        lr = wgt * fma(am.x, sr, am.y * si);
-        li = wgt * fma(am.x, si, -am.y * sr);
+        lr = wgt * fma(am.x, sr[iw], am.y * si[iw]);
+        li = wgt * fma(am.x, si[iw], -am.y * sr[iw]);
       }
     }
     if (N1LOG)
