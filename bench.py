@@ -1,15 +1,16 @@
 """Benchmark of the GPP sigma sum on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-Workload: the canonical mini-app shape number_bands=512, nvband=2,
-ncouls=32768, nodes_per_group=20 (ngpown=1638), seeded random inputs
-(generator config 2), resident in HBM before timing.  One step = one full
-sigma evaluation (prepass, non-occupied kernel, occupied kernel, static
-remainder, fixed-order reduction, 12-double read-back) through the C ABI on
-torch's current stream.  The FP64 path and the mixed path are both timed;
-`value` is the throughput of the fastest path whose achtemp/asxtemp entries
-are all within 1e-8 relative of the FP64 oracle on the checked band slice.
+Default workload (the headline line): the canonical mini-app shape
+number_bands=512, nvband=2, ncouls=32768, nodes_per_group=20 (ngpown=1638),
+seeded random inputs (generator config 2), resident in HBM before timing.
+One step = one full sigma evaluation (prepass, non-occupied kernel,
+occupied kernel, static remainder, fixed-order reduction, read-back of the
+results) through the C ABI on torch's current stream.  The FP64 path and
+the mixed path are both timed; `value` is the throughput of the fastest path
+whose achtemp/asxtemp entries are all within 1e-8 relative of the FP64
+oracle on the checked slice.
 
 The JSON line also carries:
   roofline      the non-occupied kernel (the dominant one): algorithmic
@@ -20,6 +21,17 @@ The JSON line also carries:
   cpu_baseline  the reference mini-app (oracle/_ref/gppKer_xl, built from
                 gppKerOpenMPFor_xl.cpp) on a band subset of the same shape,
                 all host cores, its own chrono timer; terms/s.
+
+Other workloads (their own JSON line, no CPU baseline):
+  band_resolved  512/2/32768/20 with band-resolved energies and 5% of the
+                 wtilde real parts within 1e-4 of a frequency (config 3)
+  large          2048/16/65536/20, config-3 values, 11.2 GB generated in
+                 device memory
+  batch          64 independent config-3 instances (ncouls log-uniform in
+                 [2048, 32768], number_bands in {128,256,512,1024}, nvband in
+                 [2,16], nodes_per_group 20) in one persistent launch captured
+                 in a hipGraph, tile autotuned, against running them one by
+                 one with the FP64 path
 """
 import argparse
 import json
@@ -36,6 +48,16 @@ ROOT = Path(__file__).resolve().parent
 NB, NV, NC, NPG = 512, 2, 32768, 20
 FLOP_UNOCC, FLOP_OCC = 25.0, 45.0
 PEAK_FP64_TFLOPS = 78.6
+METRIC = "GPP sigma terms/s on 1 MI355X, mixed vs FP64 @<=1e-8 rel err; % VALU peak"
+
+WORKLOADS = {
+    "canonical": dict(config=2, nb=512, nv=2, nc=32768, npg=20, gen="host",
+                      desc="GPP sigma sum, canonical gppKer shape, seeded random inputs"),
+    "band_resolved": dict(config=3, nb=512, nv=2, nc=32768, npg=20, gen="host",
+                          desc="GPP sigma sum, band-resolved energies, 5% near-resonant wtilde"),
+    "large": dict(config=3, nb=2048, nv=16, nc=65536, npg=20, gen="device",
+                  desc="GPP sigma sum, large single instance, band-resolved values"),
+}
 
 
 def _load(name, path, pkg=False):
@@ -60,6 +82,12 @@ def host_cores() -> int:
     except Exception:
         n = os.cpu_count() or 1
     return max(1, min(n, 16))
+
+
+def rel_err(a, b) -> float:
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.max(np.abs(a - b) / np.abs(b)))
 
 
 def cpu_baseline(args):
@@ -90,12 +118,232 @@ def cpu_baseline(args):
     gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
     orc = _load("gpp_oracle", ROOT / "oracle" / "oracle.py")
     inp = gppk.fill_inputs(2, 3, NC, NPG, nvband=NV, seed=1)
-    os.environ["OMP_NUM_THREADS"] = str(cores)
     t0 = time.perf_counter()
     orc.sigma(inp, want_achstemp=False)
     t = time.perf_counter() - t0
     return {"value": 3 * ng * NC * 3 / t, "unit": "terms/s", "cores": cores,
             "kind": "port", "sample": f"CPU oracle, 3 bands of 512 {NV} {NC} {NPG}"}
+
+
+def to_host(d: dict) -> dict:
+    out = {}
+    for k, v in d.items():
+        out[k] = v.cpu().numpy() if hasattr(v, "cpu") else v
+    return out
+
+
+def check_slice(gppk, h, inp_host: dict, ng_slice: int | None, nb_slice: int):
+    """Oracle check on bands [0, nb_slice) and, if given, the first
+    ng_slice G' columns.  Returns {precision: max rel err}."""
+    orc = _load("gpp_oracle", ROOT / "oracle" / "oracle.py")
+    sub = gppk.band_slice(inp_host, 0, nb_slice)
+    if ng_slice is not None:
+        sub = dict(sub, I_eps_array=sub["I_eps_array"][:ng_slice],
+                   wtilde_array=sub["wtilde_array"][:ng_slice],
+                   inv_igp_index=sub["inv_igp_index"][:ng_slice], ngpown=ng_slice)
+    sub = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v)
+           for k, v in sub.items()}
+    ref = orc.sigma(sub, want_achstemp=False)
+    errs = {}
+    for prec in ("fp64", "mixed"):
+        g = h.sigma(sub, precision=prec)
+        errs[prec] = max(rel_err(g[k], ref[k]) for k in ("achtemp", "asxtemp"))
+    return errs
+
+
+def time_path(h, dev, prec, steps, warmup, stream, gppk):
+    call = lambda: h.sigma(dev, precision=prec, memory=gppk.MEM_DEVICE,
+                           stream=stream.cuda_stream)
+    import torch
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = [call() for _ in range(steps)]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    return dt, res
+
+
+def run_single(args, name):
+    import torch
+    w = WORKLOADS[name]
+    gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
+    nb, nv, nc, npg = w["nb"], w["nv"], w["nc"], w["npg"]
+    ng = nc // npg
+    log(f"{name}: generating inputs {nb} {nv} {nc} {npg} config {w['config']} "
+        f"seed {args.seed} on {w['gen']}")
+    if w["gen"] == "host":
+        inp = gppk.fill_inputs(w["config"], nb, nc, npg, nvband=nv, seed=args.seed)
+        dev = {k: (torch.from_numpy(np.ascontiguousarray(v)).cuda()
+                   if isinstance(v, np.ndarray) else v) for k, v in inp.items()}
+    else:
+        dev = gppk.fill_inputs_device(w["config"], nb, nc, npg, nvband=nv,
+                                      seed=args.seed)
+        inp = None
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    h = gppk.Handle(0)
+    terms = nb * ng * nc * 3
+    unocc_terms = (nb - nv) * ng * nc * 3
+    steps = args.steps if name != "large" else min(args.steps, 3)
+    warmup = args.warmup if name != "large" else min(args.warmup, 1)
+
+    runs = {}
+    for prec in args.precisions.split(","):
+        dt, res = time_path(h, dev, prec, steps, warmup, stream, gppk)
+        ku = float(np.mean([r["unocc_kernel_ms"] for r in res]))
+        ko = float(np.mean([r["occ_kernel_ms"] for r in res]))
+        runs[prec] = {"ms_per_step": dt * 1e3, "terms_per_s": terms / dt,
+                      "unocc_kernel_ms": ku, "occ_kernel_ms": ko,
+                      "achtemp": res[-1]["achtemp"], "asxtemp": res[-1]["asxtemp"],
+                      "fallback": res[-1]["fp64_fallback_terms"]}
+        log(f"{prec}: {dt*1e3:.3f} ms/step, {terms/dt:.4e} terms/s, "
+            f"unocc kernel {ku:.3f} ms, occ kernel {ko:.3f} ms, "
+            f"FP64-redone terms {res[-1]['fp64_fallback_terms']}")
+
+    # Error of each path against the FP64 oracle on a slice (all occupied
+    # bands and one unoccupied band; for the large instance only its first
+    # 128 G' columns), and of mixed against FP64 on the full problem,
+    # per achtemp/asxtemp entry.
+    errs = {}
+    if not args.no_check:
+        host = inp if inp is not None else to_host(
+            {k: (v[: nv + 1] if k in ("aqsmtemp", "aqsntemp") or
+                 (k == "wx_array" and v.ndim == 2) else
+                 v[:128] if k in ("I_eps_array", "wtilde_array", "inv_igp_index") else v)
+             for k, v in dev.items()})
+        e = check_slice(gppk, h, host, None if inp is not None else 128, nv + 1)
+        for p in runs:
+            errs[p] = e[p]
+            log(f"{p}: max rel err vs oracle on the slice {e[p]:.3e}")
+        if "fp64" in runs and "mixed" in runs:
+            per = {k: (np.abs(runs["mixed"][k] - runs["fp64"][k]) /
+                       np.abs(runs["fp64"][k])).tolist() for k in ("achtemp", "asxtemp")}
+            errs["mixed_vs_fp64_full"] = max(max(v) for v in per.values())
+            errs["mixed_vs_fp64_full_per_entry"] = per
+            log(f"mixed vs fp64 full problem: {errs['mixed_vs_fp64_full']:.3e}")
+
+    ok = [p for p in runs if errs.get(p, 0.0) <= 1e-8 and
+          (p != "mixed" or errs.get("mixed_vs_fp64_full", 0.0) <= 1e-8)]
+    best = max(ok, key=lambda p: runs[p]["terms_per_s"]) if ok else "fp64"
+    b = runs[best]
+    achieved = FLOP_UNOCC * unocc_terms / (b["unocc_kernel_ms"] * 1e-3) / 1e12
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_unocc_{best}_{nb}_{nv}_{nc}_{npg}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": b["terms_per_s"],
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": b["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "mixed(f32 terms, f64 sums)",
+        "data": "synthetic",
+        "config": {"workload": w["desc"], "number_bands": nb, "nvband": nv,
+                   "ncouls": nc, "nodes_per_group": npg, "ngpown": ng,
+                   "terms": terms, "path": best},
+        "roofline": {"bound": "valu", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "kernel": f"unocc_{best}",
+                     "flops_per_launch": FLOP_UNOCC * unocc_terms,
+                     "avg_launch_ms": b["unocc_kernel_ms"]},
+        "paths": {p: {"terms_per_s": r["terms_per_s"], "ms_per_step": r["ms_per_step"],
+                      "unocc_kernel_ms": r["unocc_kernel_ms"],
+                      "occ_kernel_ms": r["occ_kernel_ms"],
+                      "valu_frac_fp64_peak": FLOP_UNOCC * unocc_terms /
+                      (r["unocc_kernel_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                      "fp64_redone_terms": r["fallback"],
+                      "max_rel_err_vs_oracle": errs.get(p)} for p, r in runs.items()},
+        "mixed_vs_fp64_full_rel_err": errs.get("mixed_vs_fp64_full"),
+        "mixed_vs_fp64_full_rel_err_per_entry": errs.get("mixed_vs_fp64_full_per_entry"),
+    }
+    if "fp64" in runs and "mixed" in runs:
+        line["mixed_speedup_vs_fp64"] = runs["mixed"]["terms_per_s"] / runs["fp64"]["terms_per_s"]
+    if name == "canonical" and not args.no_cpu:
+        log("timing the CPU baseline")
+        line["cpu_baseline"] = cpu_baseline(args)
+    h.close()
+    return line
+
+
+def run_batch(args):
+    import torch
+    gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
+    rng = np.random.default_rng(args.seed)
+    insts = []
+    for i in range(64):
+        nc = int(round(np.exp(rng.uniform(np.log(2048), np.log(32768)))))
+        nb = int(rng.choice([128, 256, 512, 1024]))
+        nv = int(rng.integers(2, 17))
+        insts.append(gppk.fill_inputs_device(3, nb, nc, 20, nvband=nv,
+                                             seed=args.seed + 1000 + i))
+    torch.cuda.synchronize()
+    terms = sum(d["number_bands"] * d["ngpown"] * d["ncouls"] * 3 for d in insts)
+    gbytes = sum(16 * (2 * d["number_bands"] + 2 * d["ngpown"]) * d["ncouls"]
+                 for d in insts) / 1e9
+    log(f"batch: 64 instances, {terms:.4e} terms, {gbytes:.1f} GB resident")
+    stream = torch.cuda.current_stream()
+    h = gppk.Handle(0)
+    out = {}
+    for prec in args.precisions.split(","):
+        k, wv = h.autotune(max(insts, key=lambda d: d["number_bands"] * d["ngpown"] * d["ncouls"]),
+                           precision=prec, memory=gppk.MEM_DEVICE, stream=stream.cuda_stream)
+        call = lambda: h.batch(insts, precision=prec, memory=gppk.MEM_DEVICE,
+                               stream=stream.cuda_stream)
+        for _ in range(max(1, args.warmup)):
+            res = call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        steps = min(args.steps, 5)
+        for _ in range(steps):
+            res = call()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[prec] = {"ms_per_step": dt * 1e3, "terms_per_s": terms / dt,
+                     "tile_ig": k, "results": res}
+        log(f"batch {prec}: {dt*1e3:.2f} ms, {terms/dt:.4e} terms/s (tile K={k})")
+    # one by one, FP64, same handle and stream
+    for d in insts:
+        h.sigma(d, memory=gppk.MEM_DEVICE, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    single = [h.sigma(d, memory=gppk.MEM_DEVICE, stream=stream.cuda_stream) for d in insts]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter() - t0
+    log(f"one by one fp64: {t1*1e3:.2f} ms, {terms/t1:.4e} terms/s")
+    errs = {}
+    for prec in out:
+        errs[prec] = max(rel_err(r[k], s[k]) for r, s in zip(out[prec]["results"], single)
+                         for k in ("achtemp", "asxtemp"))
+        log(f"batch {prec} vs one-by-one fp64: max rel err {errs[prec]:.3e}")
+    ok = [p for p in out if errs[p] <= 1e-8]
+    best = max(ok, key=lambda p: out[p]["terms_per_s"]) if ok else "fp64"
+    line = {
+        "metric": METRIC, "value": out[best]["terms_per_s"], "unit": "terms/s",
+        "n_gpus": 1, "steps": min(args.steps, 5), "warmup": max(1, args.warmup),
+        "ms_per_step": out[best]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "mixed(f32 terms, f64 sums)",
+        "data": "synthetic",
+        "config": {"workload": "64 independent GPP sigma evaluations, one persistent "
+                               "launch in a hipGraph, autotuned tile",
+                   "instances": 64, "terms": terms, "resident_GB": gbytes, "path": best},
+        "paths": {p: {"terms_per_s": v["terms_per_s"], "ms_per_step": v["ms_per_step"],
+                      "tile_ig": v["tile_ig"], "max_rel_err_vs_fp64_one_by_one": errs[p]}
+                  for p, v in out.items()},
+        "one_by_one_fp64": {"terms_per_s": terms / t1, "ms": t1 * 1e3},
+        "batch_speedup_vs_one_by_one_fp64": out[best]["terms_per_s"] / (terms / t1),
+    }
+    h.close()
+    return line
 
 
 def main():
@@ -104,6 +352,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--workload", default="canonical",
+                    choices=list(WORKLOADS) + ["batch"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -134,109 +384,7 @@ def main():
               "(the sum would shard over G' columns with one all-reduce of 12 "
               "doubles); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-
-    import torch
-    gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
-    ng = NC // NPG
-    log(f"generating inputs {NB} {NV} {NC} {NPG} seed {args.seed}")
-    inp = gppk.fill_inputs(2, NB, NC, NPG, nvband=NV, seed=args.seed)
-    dev = {k: (torch.from_numpy(np.ascontiguousarray(v)).cuda() if isinstance(v, np.ndarray) else v)
-           for k, v in inp.items()}
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    h = gppk.Handle(0)
-    terms = NB * ng * NC * 3
-    unocc_terms = (NB - NV) * ng * NC * 3
-    occ_terms = NV * ng * NC * 3
-
-    runs = {}
-    for prec in args.precisions.split(","):
-        call = lambda: h.sigma(dev, precision=prec, memory=gppk.MEM_DEVICE,
-                               stream=stream.cuda_stream)
-        for _ in range(args.warmup):
-            call()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = []
-        for i in range(args.steps):
-            res.append(call())
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
-        ku = float(np.mean([r["unocc_kernel_ms"] for r in res]))
-        ko = float(np.mean([r["occ_kernel_ms"] for r in res]))
-        runs[prec] = {"ms_per_step": dt * 1e3, "terms_per_s": terms / dt,
-                      "unocc_kernel_ms": ku, "occ_kernel_ms": ko,
-                      "achtemp": res[-1]["achtemp"], "asxtemp": res[-1]["asxtemp"],
-                      "fallback": res[-1]["fp64_fallback_terms"]}
-        log(f"{prec}: {dt*1e3:.3f} ms/step, {terms/dt:.4e} terms/s, "
-            f"unocc kernel {ku:.3f} ms, occ kernel {ko:.3f} ms")
-
-    # Error of each path against the FP64 oracle on a band slice (bands 0..3:
-    # both occupied bands and one unoccupied band at the full G-vector shape),
-    # and of mixed against FP64 on the full problem.
-    errs = {}
-    if not args.no_check:
-        orc = _load("gpp_oracle", ROOT / "oracle" / "oracle.py")
-        sub = gppk.band_slice(inp, 0, 3)
-        ref = orc.sigma(sub, want_achstemp=False)
-        for prec in runs:
-            g = h.sigma(sub, precision=prec)
-            e = max(float(np.max(np.abs(g[k] - ref[k]) / np.abs(ref[k])))
-                    for k in ("achtemp", "asxtemp"))
-            errs[prec] = e
-            log(f"{prec}: max rel err vs oracle (bands 0-2) {e:.3e}")
-        if "fp64" in runs and "mixed" in runs:
-            e = max(float(np.max(np.abs(runs["mixed"][k] - runs["fp64"][k]) /
-                                 np.abs(runs["fp64"][k]))) for k in ("achtemp", "asxtemp"))
-            errs["mixed_vs_fp64_full"] = e
-            log(f"mixed vs fp64 full problem: {e:.3e}")
-
-    ok = [p for p in runs if errs.get(p, 0.0) <= 1e-8 and
-          (p != "mixed" or errs.get("mixed_vs_fp64_full", 0.0) <= 1e-8)]
-    best = max(ok, key=lambda p: runs[p]["terms_per_s"]) if ok else "fp64"
-    b = runs[best]
-    ku = b["unocc_kernel_ms"] * 1e-3
-    achieved = FLOP_UNOCC * unocc_terms / ku / 1e12
-    traffic = None
-    tf = ROOT / "profiles" / f"traffic_unocc_{best}_512_2_32768_20.json"
-    if tf.exists():
-        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
-    line = {
-        "metric": "GPP sigma terms/s on 1 MI355X, mixed vs FP64 @<=1e-8 rel err; % VALU peak",
-        "value": b["terms_per_s"],
-        "unit": "terms/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": b["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "mixed(f32 terms, f64 sums)",
-        "data": "synthetic",
-        "config": {"workload": "GPP sigma sum, canonical gppKer shape, seeded random inputs",
-                   "number_bands": NB, "nvband": NV, "ncouls": NC,
-                   "nodes_per_group": NPG, "ngpown": ng, "terms": terms,
-                   "path": best},
-        "roofline": {"bound": "valu", "achieved": achieved,
-                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
-                     "kernel": f"unocc_{best}",
-                     "flops_per_launch": FLOP_UNOCC * unocc_terms,
-                     "avg_launch_ms": b["unocc_kernel_ms"]},
-        "paths": {p: {"terms_per_s": r["terms_per_s"], "ms_per_step": r["ms_per_step"],
-                      "unocc_kernel_ms": r["unocc_kernel_ms"],
-                      "occ_kernel_ms": r["occ_kernel_ms"],
-                      "valu_frac_fp64_peak": FLOP_UNOCC * unocc_terms / (r["unocc_kernel_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
-                      "max_rel_err_vs_oracle": errs.get(p)} for p, r in runs.items()},
-        "mixed_vs_fp64_full_rel_err": errs.get("mixed_vs_fp64_full"),
-    }
-    if "fp64" in runs and "mixed" in runs:
-        line["mixed_speedup_vs_fp64"] = runs["mixed"]["terms_per_s"] / runs["fp64"]["terms_per_s"]
-    if not args.no_cpu:
-        log("timing the CPU baseline")
-        line["cpu_baseline"] = cpu_baseline(args)
-    h.close()
+    line = run_batch(args) if args.workload == "batch" else run_single(args, args.workload)
     print(json.dumps(line), flush=True)
 
 

@@ -50,7 +50,7 @@ EXPORTS = (
     "gpp_create", "gpp_destroy", "gpp_last_error", "gpp_default_options",
     "gpp_sigma_ex", "gpp_sigma_batch", "gpp_autotune", "gpp_sigma",
     "gpp_achtemp_kernel", "gpp_tillNvband_kernel", "gpp_fill_inputs",
-    "gpp_version",
+    "gpp_fill_inputs_device", "gpp_version",
 )
 
 
@@ -119,6 +119,8 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     lib.gpp_tillNvband_kernel.argtypes = ([vp] * 8 + [ip, ip, ip, vp, ip, ip,
                                           C.c_double, ip])
     lib.gpp_fill_inputs.argtypes = [ip, C.c_ulonglong, ip, ip, ip, ip] + [vp] * 8
+    lib.gpp_fill_inputs_device.argtypes = ([ip, C.c_ulonglong, ip, ip, ip, ip]
+                                           + [vp] * 9)
     lib.gpp_version.restype = C.c_char_p
     if path is None:
         _lib = lib
@@ -305,6 +307,41 @@ def fill_inputs(config: int, number_bands: int, ncouls: int,
     if rc != GPP_OK:
         raise GppError(rc, "gpp_fill_inputs failed")
     d["inv_igp_index"] = d["inv_igp_index"][:ng].copy()
+    d.update(number_bands=nb, ncouls=nc, ngpown=ng, nvband=nvband)
+    return d
+
+
+def fill_inputs_device(config: int, number_bands: int, ncouls: int,
+                       nodes_per_group: int, nvband: int, seed: int = 12345,
+                       lib: C.CDLL | None = None) -> dict:
+    """The same synthetic inputs generated in device memory (torch tensors on
+    cuda:0, complex128 / float64 / int32), by gpp_fill_inputs_device."""
+    import torch
+    lib = lib or load_library()
+    nb, nc = number_bands, ncouls
+    ng = nc // nodes_per_group
+    dev = torch.device("cuda", 0)
+    d = {
+        "aqsmtemp": torch.empty((nb, nc), dtype=torch.complex128, device=dev),
+        "aqsntemp": torch.empty((nb, nc), dtype=torch.complex128, device=dev),
+        "I_eps_array": torch.empty((ng, nc), dtype=torch.complex128, device=dev),
+        "wtilde_array": torch.empty((ng, nc), dtype=torch.complex128, device=dev),
+        "vcoul": torch.empty(nc, dtype=torch.float64, device=dev),
+        "inv_igp_index": torch.empty(max(ng, 1), dtype=torch.int32, device=dev),
+        "indinv": torch.empty(nc, dtype=torch.int32, device=dev),
+        "wx_array": torch.empty((nb, 3) if config == 3 else 3,
+                                dtype=torch.float64, device=dev),
+    }
+    stream = torch.cuda.current_stream().cuda_stream
+    rc = lib.gpp_fill_inputs_device(config, seed, nb, nc, ng, nodes_per_group,
+                                    *[d[k].data_ptr() for k in (
+                                        "aqsmtemp", "aqsntemp", "I_eps_array",
+                                        "wtilde_array", "vcoul",
+                                        "inv_igp_index", "indinv", "wx_array")],
+                                    stream)
+    if rc != GPP_OK:
+        raise GppError(rc, "gpp_fill_inputs_device failed")
+    d["inv_igp_index"] = d["inv_igp_index"][:ng]
     d.update(number_bands=nb, ncouls=nc, ngpown=ng, nvband=nvband)
     return d
 

@@ -22,39 +22,16 @@
 #include <vector>
 
 #include "../../include/gpp_sigma.h"
+#include "gpp_rng.h"
 
 namespace {
 
-enum Stream : uint64_t {
-  kAqsm = 1,
-  kAqsn = 2,
-  kIeps = 3,
-  kWtilde = 4,
-  kEnergy = 5,
-  kResPick = 6,
-  kResWhere = 7,
-};
-
-inline uint64_t mix64(uint64_t z) {
-  // SplitMix64 finaliser.
-  z += 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
-inline uint64_t draw(uint64_t seed, uint64_t stream, uint64_t i) {
-  const uint64_t key = mix64(seed * 0xD1B54A32D192ED03ull + stream);
-  return mix64(key ^ mix64(i));
-}
-
-// [0,1) with 53 random bits.
-inline double u01(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }
+using namespace gpp::rng;
 
 // Standard normal pair by Box-Muller from draws 2i and 2i+1.
 inline void normal2(uint64_t seed, uint64_t stream, uint64_t i, double *a,
                     double *b) {
-  const double u1 = ((draw(seed, stream, 2 * i) >> 11) + 1) * 0x1.0p-53;
+  const double u1 = u01_open0(draw(seed, stream, 2 * i));
   const double u2 = u01(draw(seed, stream, 2 * i + 1));
   const double r = std::sqrt(-2.0 * std::log(u1));
   const double t = 6.283185307179586 * u2;

@@ -206,6 +206,20 @@ int gpp_fill_inputs(int config, unsigned long long seed, int number_bands,
                     double *vcoul, int *inv_igp_index, int *indinv,
                     double *wx_array);
 
+/* The same inputs generated in device memory (all pointers are device
+ * pointers on the current device; stream is a hipStream_t or NULL).  Returns
+ * after the stream has finished.  I_eps, wtilde, vcoul, the index tables and
+ * wx are bit-identical to gpp_fill_inputs; aqsm/aqsn (normal variates) use
+ * the device's log/sin/cos and agree with the host's to a few ulp.  For
+ * shapes too large to stage from the host on every run. */
+int gpp_fill_inputs_device(int config, unsigned long long seed,
+                           int number_bands, int ncouls, int ngpown,
+                           int nodes_per_group, gpp_c128 *aqsmtemp,
+                           gpp_c128 *aqsntemp, gpp_c128 *I_eps_array,
+                           gpp_c128 *wtilde_array, double *vcoul,
+                           int *inv_igp_index, int *indinv, double *wx_array,
+                           void *stream);
+
 /* Version string of the library build. */
 const char *gpp_version(void);
 

@@ -286,3 +286,33 @@ def test_autotune_picks_valid_tile_and_agrees(handle):
     ref = orc.sigma(inp)
     check(handle.sigma(inp), ref)            # uses the tuned tile
     check(handle.sigma(inp, flags=gppk.FLAG_AUTOTUNE), ref)
+
+
+@pytest.mark.parametrize("cfg", [1, 2, 3])
+def test_device_generator_matches_host(handle, cfg):
+    import torch
+    say("device generator", cfg)
+    args = (cfg, 7, 1000, 10)
+    host = gppk.fill_inputs(*args, nvband=2, seed=77)
+    dev = gppk.fill_inputs_device(*args, nvband=2, seed=77)
+    torch.cuda.synchronize()
+    for k in ("I_eps_array", "wtilde_array", "vcoul", "inv_igp_index", "indinv", "wx_array"):
+        assert np.array_equal(dev[k].cpu().numpy(), host[k]), k
+    for k in ("aqsmtemp", "aqsntemp"):
+        d = dev[k].cpu().numpy()
+        assert np.allclose(d, host[k], rtol=1e-13, atol=1e-16), k
+    a = handle.sigma(host)
+    b = handle.sigma(dev, memory=gppk.MEM_DEVICE)
+    check(b, a, tol=1e-12)
+
+
+def test_batch_device_memory(handle):
+    import torch
+    say("batch device memory")
+    insts = [gppk.fill_inputs_device(3, nb, nc, 8, nvband=nv, seed=300 + i)
+             for i, (nb, nc, nv) in enumerate([(6, 512, 2), (9, 777, 3), (4, 2048, 1)])]
+    torch.cuda.synchronize()
+    res = handle.batch(insts, memory=gppk.MEM_DEVICE)
+    for d, r in zip(insts, res):
+        host = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in d.items()}
+        check(r, orc.sigma(host))
