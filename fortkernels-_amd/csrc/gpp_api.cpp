@@ -684,12 +684,18 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   {
     std::vector<int> order(items.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return cost[a] > cost[b]; });
+    // Non-occupied items first, then occupied ones; largest first in each.
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+      if (items[a].kind != items[b].kind) return items[a].kind < items[b].kind;
+      return cost[a] > cost[b];
+    });
     std::vector<BatchItem> sorted(items.size());
     for (size_t i = 0; i < order.size(); ++i) sorted[i] = items[order[i]];
     items.swap(sorted);
   }
+  int n_u = 0;
+  while (n_u < (int)items.size() && items[n_u].kind == 0) ++n_u;
+  const int n_o = (int)items.size() - n_u;
   int nach = 0;
   for (int i = 0; i < n; ++i) {
     ach_begin[i] = nrec + nach;
@@ -731,13 +737,17 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   double *dout_ach = dout + (size_t)n * gpp::kSlots;
   if ((rc = ensure_host_out(h, (size_t)2 * n * gpp::kSlots + 1))) return rc;
 
-  const int per_cu = gpp::batch_blocks_per_cu(tile.k_ig, o->precision,
-                                              o->semantics);
   hipDeviceProp_t prop;
   HIP_OR_RET(h, hipGetDeviceProperties(&prop, h->device));
-  int grid = prop.multiProcessorCount * per_cu;
-  if (grid > (int)items.size()) grid = (int)items.size();
-  if (grid < 1) grid = 1;
+  auto grid_for = [&](int kind, int nitems) {
+    int g = prop.multiProcessorCount *
+            gpp::batch_blocks_per_cu(tile.k_ig, o->precision, o->semantics,
+                                     kind);
+    if (g > nitems) g = nitems;
+    return g < 1 ? 1 : g;
+  };
+  const int grid_u = grid_for(0, n_u), grid_o = grid_for(1, n_o);
+  const int grid = grid_u * 65536 + grid_o;  // part of the graph signature
   const double thr =
       o->resonance_threshold > 0 ? o->resonance_threshold : kDefaultResonance;
 
@@ -767,11 +777,12 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     for (int i = 0; i < n && crc == GPP_OK; ++i)
       crc = enqueue_prepass(h, prep[i], mixed, dcount, s);
     if (crc == GPP_OK) {
+      const BatchItem *ditems = (const BatchItem *)(mb + off_items);
       hipError_t e = gpp::launch_batch(
-          (const DevProblem *)(mb + off_tab),
-          (const BatchItem *)(mb + off_items), (int)items.size(), tile.k_ig,
-          o->precision, o->semantics, thr, part, work_counter, dcount, grid,
-          (const int *)(mb + off_rb), (const int *)(mb + off_rc), n, dout, s);
+          (const DevProblem *)(mb + off_tab), ditems, n_u, ditems + n_u, n_o,
+          tile.k_ig, o->precision, o->semantics, thr, part, work_counter,
+          dcount, grid_u, grid_o, (const int *)(mb + off_rb),
+          (const int *)(mb + off_rc), n, dout, s);
       if (e != hipSuccess)
         crc = set_err(h, GPP_EDEVICE, "batch launch: %s", hipGetErrorString(e));
     }

@@ -22,6 +22,10 @@ using namespace gpp::rng;
 
 constexpr int kThreads = 256;
 
+// The host generator evaluates a + b*u as a rounded multiply then a rounded
+// add; so must the device (no fused multiply-add) to give the same bits.
+#pragma clang fp contract(off)
+
 unsigned grid_for(size_t n) {
   size_t b = (n + kThreads - 1) / kThreads;
   if (b > 65536) b = 65536;

@@ -80,15 +80,20 @@ hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
                            hipStream_t s);
 // Batched persistent launch (block size 256 = 4 waves for the non-occupied
 // items) followed by one reduction block per problem into out[prob][12].
-int batch_blocks_per_cu(int k_ig, int precision, int sem);
+// kind 0 = non-occupied items, 1 = occupied items.
+int batch_blocks_per_cu(int k_ig, int precision, int sem, int kind);
 hipError_t launch_reduce_batch(const double *partials, const int *rec_begin,
                                const int *rec_count, int nprob, double *out,
                                hipStream_t s);
-hipError_t launch_batch(const DevProblem *probs, const BatchItem *items,
-                        int nitems, int k_ig, int precision, int sem,
-                        double res_thresh, double *partials, int *counter,
-                        unsigned long long *fallback_count, int grid,
-                        const int *rec_begin, const int *rec_count, int nprob,
-                        double *out, hipStream_t s);
+// Two persistent kernels (non-occupied items, then occupied items), each
+// draining its own list with its own counter (counters[0], counters[1]).
+hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
+                        int n_u, const BatchItem *items_o, int n_o, int k_ig,
+                        int precision, int sem, double res_thresh,
+                        double *partials, int *counters,
+                        unsigned long long *fallback_count, int grid_u,
+                        int grid_o, const int *rec_begin,
+                        const int *rec_count, int nprob, double *out,
+                        hipStream_t s);
 
 }  // namespace gpp

@@ -139,10 +139,12 @@ void gpp_default_options(gpp_options *o);
 int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
                  gpp_result *r);
 
-/* Many independent evaluations in one persistent launch (FP64 or mixed),
- * captured in a hipGraph that is replayed while the batch (pointers, shapes,
- * options) is unchanged.  All problems use the same options and memory
- * space; band-range flags and acht_n1_loc are not available here. */
+/* Many independent evaluations of uneven size (FP64 or mixed) in one
+ * launch per band class: the blocks of all problems form one work list,
+ * sorted largest-first, run by a single grid; the sequence is captured in a
+ * hipGraph that is replayed while the batch (pointers, shapes, options) is
+ * unchanged.  All problems use the same options and memory space;
+ * band-range flags and acht_n1_loc are not available here. */
 int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
                     const gpp_options *o, gpp_result *r);
 
