@@ -627,8 +627,6 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   if (o->flags & (GPP_FLAG_ALL_BANDS_UNOCC | GPP_FLAG_SKIP_UNOCC))
     return set_err(h, GPP_EUNSUPPORTED,
                    "band-range flags are not supported in batches");
-  if (o->waves_per_block && o->waves_per_block != 4)
-    return set_err(h, GPP_EINVAL, "batched launch uses 4 waves per block");
   HIP_OR_RET(h, hipSetDevice(h->device));
   hipStream_t s = (o->memory == GPP_MEM_DEVICE && o->stream)
                       ? (hipStream_t)o->stream
@@ -642,19 +640,14 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     if ((double)p[i].number_bands * p[i].ngpown * p[i].ncouls >
         (double)p[big].number_bands * p[big].ngpown * p[big].ncouls)
       big = i;
-  TileCfg tile = default_tile(o->precision);
-  if (o->tile_ig) {
-    tile.k_ig = o->tile_ig;
-  } else {
-    auto it = h->tuned.find(tune_key(p + big, o));
-    if (it == h->tuned.end() && (o->flags & GPP_FLAG_AUTOTUNE)) {
-      int k, w;
-      if ((rc = gpp_autotune(h, p + big, o, &k, &w))) return rc;
-      it = h->tuned.find(tune_key(p + big, o));
-    }
-    if (it != h->tuned.end()) tile.k_ig = it->second.k_ig;
+  if ((o->flags & GPP_FLAG_AUTOTUNE) && !o->tile_ig && !o->waves_per_block &&
+      !h->tuned.count(tune_key(p + big, o))) {
+    int k, w;
+    if ((rc = gpp_autotune(h, p + big, o, &k, &w))) return rc;
   }
-  tile.waves = 4;
+  // The occupied items always run 256-thread blocks; the non-occupied ones
+  // use the tile's waves per block.
+  const TileCfg tile = pick_tile(h, p + big, o);
 
   std::vector<Prepared> prep;
   if ((rc = prepare(h, p, n, o, mixed, s, prep))) return rc;
@@ -669,7 +662,7 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     rec_begin[i] = nrec;
     const int nu = dp.NB > dp.NV ? gpp::unocc_grid(dp, tile) : 0;
     const int no = dp.NV > 0 ? gpp::occ_grid(dp) : 0;
-    const double cu = (double)(dp.NB - dp.NV) * tile.k_ig * 4 * 64;
+    const double cu = (double)(dp.NB - dp.NV) * tile.k_ig * tile.waves * 64;
     const double co = (double)dp.NV * std::min(dp.NC, 2048) * 5.0;
     for (int b = 0; b < nu; ++b) {
       items.push_back(BatchItem{i, 0, b, nrec++});
@@ -781,7 +774,7 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
       hipError_t e = gpp::launch_batch(
           (const DevProblem *)(mb + off_tab), ditems, n_u, ditems + n_u, n_o,
           tile.k_ig, o->precision, o->semantics, thr, part, work_counter,
-          dcount, grid_u, grid_o, (const int *)(mb + off_rb),
+          dcount, 64 * tile.waves, grid_o, (const int *)(mb + off_rb),
           (const int *)(mb + off_rc), n, dout, s);
       if (e != hipSuccess)
         crc = set_err(h, GPP_EDEVICE, "batch launch: %s", hipGetErrorString(e));

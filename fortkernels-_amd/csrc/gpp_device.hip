@@ -1082,12 +1082,13 @@ hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
                         int n_u, const BatchItem *items_o, int n_o, int k_ig,
                         int precision, int sem, double res_thresh,
                         double *partials, int *counters,
-                        unsigned long long *fallback_count, int grid_u,
+                        unsigned long long *fallback_count, int unocc_threads,
                         int grid_o, const int *rec_begin,
                         const int *rec_count, int nprob, double *out,
                         hipStream_t s) {
-  (void)grid_u;
   (void)grid_o;
+  if (unocc_threads < 64 || unocc_threads > 256 || unocc_threads % 64)
+    return hipErrorInvalidValue;
   if (precision == 1 && sem == kSemBgw) return hipErrorNotSupported;
   batch_reset_kernel<<<1, 64, 0, s>>>(counters, fallback_count);
   hipError_t e = hipGetLastError();
@@ -1097,16 +1098,16 @@ hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
 #define GPP_BL(KK)                                                          \
   case KK:                                                                  \
     if (precision == 1)                                                     \
-      batch_kernel<KK, 1, kSemGppker, 0><<<n_u, 256, 0, s>>>(               \
+      batch_kernel<KK, 1, kSemGppker, 0><<<n_u, unocc_threads, 0, s>>>(               \
           probs, items_u, partials, thr, fallback_count);                   \
     else if (sem == kSemGppkerStrict)                                       \
-      batch_kernel<KK, 0, kSemGppkerStrict, 0><<<n_u, 256, 0, s>>>(         \
+      batch_kernel<KK, 0, kSemGppkerStrict, 0><<<n_u, unocc_threads, 0, s>>>(         \
           probs, items_u, partials, thr, fallback_count);                   \
     else if (sem == kSemBgw)                                                \
-      batch_kernel<KK, 0, kSemBgw, 0><<<n_u, 256, 0, s>>>(                  \
+      batch_kernel<KK, 0, kSemBgw, 0><<<n_u, unocc_threads, 0, s>>>(                  \
           probs, items_u, partials, thr, fallback_count);                   \
     else                                                                    \
-      batch_kernel<KK, 0, kSemGppker, 0><<<n_u, 256, 0, s>>>(               \
+      batch_kernel<KK, 0, kSemGppker, 0><<<n_u, unocc_threads, 0, s>>>(               \
           probs, items_u, partials, thr, fallback_count);                   \
     break;
     switch (k_ig) {

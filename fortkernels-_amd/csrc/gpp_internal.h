@@ -91,7 +91,7 @@ hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
                         int n_u, const BatchItem *items_o, int n_o, int k_ig,
                         int precision, int sem, double res_thresh,
                         double *partials, int *counters,
-                        unsigned long long *fallback_count, int grid_u,
+                        unsigned long long *fallback_count, int unocc_threads,
                         int grid_o, const int *rec_begin,
                         const int *rec_count, int nprob, double *out,
                         hipStream_t s);
