@@ -226,13 +226,41 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
   double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NG]
                                         : make_double2(0.0, 0.0);
 
+  // For K <= 8 the next band's aqsn run and frequencies (wave-uniform, in
+  // SGPRs) are loaded a whole band ahead, so no scalar-load latency sits
+  // between the bands; for larger K (or the masked variant's extra live
+  // values) the registers do not hold two runs.
+  constexpr bool kAhead = K <= 8 && !MASK;
+  double2 an_cur[K];
+  double x_cur[3] = {0.0, 0.0, 0.0};
+  if (kAhead && n1_begin < n1_end) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      an_cur[k] = p.aqsn[(size_t)n1_begin * NC + min(ig0 + k, NC - 1)];
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) x_cur[iw] = p.wx[n1_begin * 3 + iw];
+  }
+
   for (int n1 = n1_begin; n1 < n1_end; ++n1) {
     const double2 am = am_next;
+    const int n1n = min(n1 + 1, n1_end - 1);
     // Prefetch next band's aqsm (index clamped: no branch in the loop).
-    am_next = aqsm_col[(size_t)min(n1 + 1, n1_end - 1) * NG];
-    const double x0 = p.wx[n1 * 3 + 0];
-    const double x1 = p.wx[n1 * 3 + 1];
-    const double x2 = p.wx[n1 * 3 + 2];
+    am_next = aqsm_col[(size_t)n1n * NG];
+    double2 an_nxt[K];
+    double x_nxt[3];
+    if (kAhead) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        an_nxt[k] = p.aqsn[(size_t)n1n * NC + min(ig0 + k, NC - 1)];
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) x_nxt[iw] = p.wx[n1n * 3 + iw];
+    } else {
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) x_cur[iw] = p.wx[n1 * 3 + iw];
+    }
+    const double x0 = x_cur[0];
+    const double x1 = x_cur[1];
+    const double x2 = x_cur[2];
     const double2 *an_row = p.aqsn + (size_t)n1 * NC;
 
     double s_r0 = 0.0, s_i0 = 0.0, s_r1 = 0.0, s_i1 = 0.0, s_r2 = 0.0,
@@ -240,8 +268,7 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int igk = min(ig0 + k, NC - 1);
-      const double2 an = an_row[igk];
+      const double2 an = kAhead ? an_cur[k] : an_row[min(ig0 + k, NC - 1)];
       const double Br = an.x * Ar[k] - an.y * Ai[k];
       const double Bi = an.x * Ai[k] + an.y * Ar[k];
       const double nBiwi = -Bi * wi[k];
@@ -328,6 +355,12 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
       n1log_store(wgt * fma(am.x, s_r2, am.y * s_i2),
                   wgt * fma(am.x, s_i2, -am.y * s_r2), n1log,
                   (size_t)b * (n1_end - n1_begin) + (n1 - n1_begin));
+    if (kAhead) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) an_cur[k] = an_nxt[k];
+#pragma unroll
+      for (int iw = 0; iw < 3; ++iw) x_cur[iw] = x_nxt[iw];
+    }
   }
 
   double v[6];
