@@ -230,7 +230,7 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
   // SGPRs) are loaded a whole band ahead, so no scalar-load latency sits
   // between the bands; for larger K (or the masked variant's extra live
   // values) the registers do not hold two runs.
-  constexpr bool kAhead = K <= 8 && !MASK;
+  constexpr bool kAhead = K <= 8 && !MASK && !N1LOG;
   double2 an_cur[K];
   double x_cur[3] = {0.0, 0.0, 0.0};
   if (kAhead && n1_begin < n1_end) {
