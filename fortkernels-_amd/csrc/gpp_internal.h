@@ -43,7 +43,7 @@ struct TileCfg {
   int waves = 4;          // waves per block (each takes its own k_ig run)
 };
 
-// One work item of the batched persistent launch.
+// One work item of the batched launch (block b of the grid runs item b).
 struct BatchItem {
   int prob;   // index into the problem table
   int kind;   // 0 = non-occupied block, 1 = occupied block
@@ -78,15 +78,14 @@ hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,
 int achstemp_records(const DevProblem &p, int sem);
 hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
                            hipStream_t s);
-// Batched persistent launch (block size 256 = 4 waves for the non-occupied
-// items) followed by one reduction block per problem into out[prob][12].
-// kind 0 = non-occupied items, 1 = occupied items.
+// Batched launch followed by one reduction block per problem into
+// out[prob][12].  kind 0 = non-occupied items, 1 = occupied items.
 int batch_blocks_per_cu(int k_ig, int precision, int sem, int kind);
 hipError_t launch_reduce_batch(const double *partials, const int *rec_begin,
                                const int *rec_count, int nprob, double *out,
                                hipStream_t s);
-// Two persistent kernels (non-occupied items, then occupied items), each
-// draining its own list with its own counter (counters[0], counters[1]).
+// Two flat launches (non-occupied items, then occupied items), one block per
+// item of each list; counters[0..1] are cleared by the reset kernel.
 hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
                         int n_u, const BatchItem *items_o, int n_o, int k_ig,
                         int precision, int sem, double res_thresh,
