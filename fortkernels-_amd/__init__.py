@@ -255,7 +255,8 @@ class Handle:
     def batch(self, inputs: list, precision="fp64", semantics="gppker",
               memory=None, flags=0, stream=None, resonance_threshold=0.0,
               tile_ig=0) -> list:
-        """Many independent evaluations in one persistent launch."""
+        """Many independent evaluations in one batched launch sequence
+        (gpp_sigma_batch), replayed from a hipGraph while unchanged."""
         probs = (Problem * len(inputs))(*[make_problem(d) for d in inputs])
         opt = self._options(inputs[0], precision, semantics, memory, flags,
                             stream, resonance_threshold, tile_ig, 0)

@@ -1,7 +1,7 @@
 // gpp_api.cpp — the C ABI of include/gpp_sigma.h: handle lifetime, input
-// validation, host->device staging, kernel sequencing, result read-back,
-// the batched persistent launch (captured in a hipGraph) and the tile
-// autotuner.
+// validation, host->device staging (pipelined with the kernel for host
+// inputs), kernel sequencing, result read-back, the batched launch
+// (captured in a hipGraph) and the tile autotuner.
 //
 // The call sequence per evaluation replaces the reference's band loop
 // (gppKer.cpp:270-361):
@@ -28,6 +28,7 @@
 
 #include "../../include/gpp_sigma.h"
 #include "gpp_internal.h"
+#include "gpp_stage.h"
 
 using gpp::BatchItem;
 using gpp::DevProblem;
@@ -68,6 +69,7 @@ struct gpp_handle_s {
   hipGraphExec_t graph_exec = nullptr;
   std::vector<unsigned char> graph_sig;
   std::string err;
+  gpp::Stager *stager = nullptr;  // host-input copies (created on first use)
 };
 
 namespace {
@@ -261,15 +263,36 @@ struct Prepared {
   int wx_per_band = 0;
 };
 
+gpp::Stager *stager(gpp_handle h, int *rc) {
+  *rc = GPP_OK;
+  if (!h->stager) h->stager = new (std::nothrow) gpp::Stager;
+  if (!h->stager) {
+    *rc = set_err(h, GPP_ENOMEM, "stager allocation failed");
+    return nullptr;
+  }
+  hipError_t e = h->stager->init();
+  if (e != hipSuccess) {
+    *rc = set_err(h, e == hipErrorOutOfMemory ? GPP_ENOMEM : GPP_EDEVICE,
+                  "pinned staging ring: %s", hipGetErrorString(e));
+    return nullptr;
+  }
+  return h->stager;
+}
+
 // Plans the device layout of `n` problems: staging (host inputs) in
-// h->stage, igp/wx/aqsn32 tables in h->work.  Enqueues the H2D copies.
+// h->stage, igp/wx/aqsn32 tables in h->work.  Host inputs are copied
+// through the pinned ring and `s` is made to wait for them, except I_eps
+// and wtilde when `defer_rows` is set (the caller then copies their rows
+// itself, chunk by chunk, into q.dp.ieps / q.dp.wtilde).
 int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
-            bool mixed, hipStream_t s, std::vector<Prepared> &out) {
+            bool mixed, hipStream_t s, std::vector<Prepared> &out,
+            bool defer_rows = false) {
   int rc;
   out.assign(n, Prepared());
   // Staging of host inputs.
   Arena sa;
   std::vector<size_t> soff((size_t)n * 8);
+  gpp::Stager *st = nullptr;
   if (o->memory == GPP_MEM_HOST) {
     for (int i = 0; i < n; ++i) {
       size_t bytes[8];
@@ -277,6 +300,7 @@ int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
       for (int j = 0; j < 8; ++j) soff[(size_t)i * 8 + j] = sa.take(bytes[j]);
     }
     if ((rc = ensure(h, h->stage, sa.used))) return rc;
+    if (!(st = stager(h, &rc))) return rc;
   }
   Arena wa;
   std::vector<size_t> woff((size_t)n * 4);
@@ -298,11 +322,13 @@ int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
     if (o->memory == GPP_MEM_HOST) {
       size_t bytes[8];
       array_bytes(p + i, bytes);
-      for (int j = 0; j < 8; ++j) {
+      // Smallest arrays first, then aqsm and aqsn, then the G' rows.
+      static const int order[8] = {4, 5, 6, 7, 0, 1, 2, 3};
+      for (int j : order) {
         void *d = sbase + soff[(size_t)i * 8 + j];
-        if (bytes[j] && src[j])
-          HIP_OR_RET(h, hipMemcpyAsync(d, src[j], bytes[j],
-                                       hipMemcpyHostToDevice, s));
+        const bool deferred = defer_rows && (j == 2 || j == 3);
+        if (bytes[j] && src[j] && !deferred)
+          HIP_OR_RET(h, st->copy(d, src[j], bytes[j]));
         dev[j] = d;
       }
     } else {
@@ -328,11 +354,17 @@ int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
     q.dp.NC = p[i].ncouls;
     q.dp.NG = p[i].ngpown;
   }
+  if (st) HIP_OR_RET(h, st->fence(s));
   return GPP_OK;
 }
 
-// The fallback counter lives in the first 64 bytes of h->part (the batch
-// work counter in the next 64).
+// Host inputs: G' groups (of 64 rows) per pipelined chunk of I_eps/wtilde.
+// About a twelfth of the rows per chunk keeps the copy of the first chunk
+// (which nothing overlaps) short, while each chunk's launch still has
+// thousands of blocks at the canonical 512/2/32768/20 shape.
+int stage_groups(int ngrp) { return std::max(1, (ngrp + 11) / 12); }
+
+// The fallback counter lives in the first 64 bytes of h->part.
 unsigned long long *counter_ptr(gpp_handle h) {
   return (unsigned long long *)h->part.ptr;
 }
@@ -430,6 +462,7 @@ int gpp_destroy(gpp_handle h) {
   for (auto &ev : h->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (h->host_out) (void)hipHostFree(h->host_out);
+  delete h->stager;
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return GPP_OK;
@@ -464,15 +497,21 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   }
   const TileCfg tile = pick_tile(h, p, o);
 
-  std::vector<Prepared> prep;
-  if ((rc = prepare(h, p, 1, o, mixed, s, prep))) return rc;
-  const DevProblem &dp = prep[0].dp;
-  const int NB = dp.NB, NV = dp.NV;
-
+  const int NB = p->number_bands, NV = std::min(p->nvband, NB);
   const int n1u_begin = all_unocc ? 0 : NV;
   const int n1o_end = all_unocc ? 0 : NV;
   const bool run_unocc = !skip_unocc && n1u_begin < NB;
   const bool run_occ = n1o_end > 0;
+  // Host inputs: I_eps and wtilde, most of the bytes, go over in chunks of
+  // G' rows; the non-occupied launch of each chunk overlaps the copy of the
+  // next.  Its blocks write the records the single launch would write, so the
+  // result is bit-identical to that of device-resident inputs.
+  const bool pipelined =
+      o->memory == GPP_MEM_HOST && run_unocc && p->ngpown > 0;
+
+  std::vector<Prepared> prep;
+  if ((rc = prepare(h, p, 1, o, mixed, s, prep, pipelined))) return rc;
+  const DevProblem &dp = prep[0].dp;
   const int n_unocc = run_unocc ? gpp::unocc_grid(dp, tile) : 0;
   const int n_occ = run_occ ? gpp::occ_grid(dp) : 0;
   const int n_ach = want_ach ? gpp::achstemp_records(dp, o->semantics) : 0;
@@ -502,9 +541,26 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   if (run_unocc) {
     const double thr =
         o->resonance_threshold > 0 ? o->resonance_threshold : kDefaultResonance;
-    HIP_OR_RET(h, gpp::launch_unocc(dp, n1u_begin, NB, o->precision,
-                                    o->semantics, tile, thr, part, n1u,
-                                    mixed ? dcount : nullptr, s));
+    const int ngrp = gpp::unocc_groups(dp);
+    const int step = pipelined ? stage_groups(ngrp) : ngrp;
+    gpp::Stager *st = pipelined ? stager(h, &rc) : nullptr;
+    if (pipelined && !st) return rc;
+    for (int g = 0; g < ngrp; g += step) {
+      const int cnt = std::min(step, ngrp - g);
+      if (pipelined) {
+        const size_t r0 = (size_t)g * 64;
+        const size_t r1 = std::min((size_t)dp.NG, (size_t)(g + cnt) * 64);
+        const size_t off = r0 * dp.NC, len = (r1 - r0) * dp.NC * 16;
+        HIP_OR_RET(h, st->copy((void *)(dp.ieps + off), p->I_eps_array + off,
+                               len));
+        HIP_OR_RET(h, st->copy((void *)(dp.wtilde + off),
+                               p->wtilde_array + off, len));
+        HIP_OR_RET(h, st->fence(s));
+      }
+      HIP_OR_RET(h, gpp::launch_unocc(dp, n1u_begin, NB, o->precision,
+                                      o->semantics, tile, thr, part, n1u,
+                                      mixed ? dcount : nullptr, s, g, cnt));
+    }
   }
   HIP_OR_RET(h, hipEventRecord(h->ev[2], s));
   if (run_occ)
@@ -718,29 +774,17 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   std::memcpy(meta_host.data() + off_ab, ach_begin.data(), sizeof(int) * n);
   std::memcpy(meta_host.data() + off_ac, ach_count.data(), sizeof(int) * n);
 
-  // h->part: [counters 256 B][records][n x 2 result slots]
+  // h->part: [counter 256 B][records][n x 2 result slots]
   const size_t rec_bytes = (size_t)(nrec + nach) * gpp::kSlots * 8;
   if ((rc = ensure(h, h->part,
                    256 + rec_bytes + (size_t)2 * n * gpp::kSlots * 8)))
     return rc;
   unsigned long long *dcount = counter_ptr(h);
-  int *work_counter = (int *)((char *)h->part.ptr + 64);
   double *part = (double *)((char *)h->part.ptr + 256);
   double *dout = part + (size_t)(nrec + nach) * gpp::kSlots;
   double *dout_ach = dout + (size_t)n * gpp::kSlots;
   if ((rc = ensure_host_out(h, (size_t)2 * n * gpp::kSlots + 1))) return rc;
 
-  hipDeviceProp_t prop;
-  HIP_OR_RET(h, hipGetDeviceProperties(&prop, h->device));
-  auto grid_for = [&](int kind, int nitems) {
-    int g = prop.multiProcessorCount *
-            gpp::batch_blocks_per_cu(tile.k_ig, o->precision, o->semantics,
-                                     kind);
-    if (g > nitems) g = nitems;
-    return g < 1 ? 1 : g;
-  };
-  const int grid_u = grid_for(0, n_u), grid_o = grid_for(1, n_o);
-  const int grid = grid_u * 65536 + grid_o;  // part of the graph signature
   const double thr =
       o->resonance_threshold > 0 ? o->resonance_threshold : kDefaultResonance;
 
@@ -752,7 +796,6 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   };
   push(o, sizeof *o);
   push(&tile, sizeof tile);
-  push(&grid, sizeof grid);
   void *bases[4] = {h->part.ptr, h->meta.ptr, h->work.ptr, (void *)s};
   push(bases, sizeof bases);
 
@@ -773,8 +816,8 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
       const BatchItem *ditems = (const BatchItem *)(mb + off_items);
       hipError_t e = gpp::launch_batch(
           (const DevProblem *)(mb + off_tab), ditems, n_u, ditems + n_u, n_o,
-          tile.k_ig, o->precision, o->semantics, thr, part, work_counter,
-          dcount, 64 * tile.waves, grid_o, (const int *)(mb + off_rb),
+          tile.k_ig, o->precision, o->semantics, thr, part, dcount,
+          64 * tile.waves, (const int *)(mb + off_rb),
           (const int *)(mb + off_rc), n, dout, s);
       if (e != hipSuccess)
         crc = set_err(h, GPP_EDEVICE, "batch launch: %s", hipGetErrorString(e));

@@ -373,13 +373,25 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
   if (threadIdx.x >= 6 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
 }
 
+// A launch may cover only the G' groups [grp_lo, grp_lo + grp_cnt) (one
+// chunk of a pipelined host->device copy).  Its blocks take the index, and
+// so the record, that the same tile has in the launch over all groups
+// (b = ig_grp * ngrp + igp_grp), so the fixed-order reduction gives the same
+// bits however the columns were split.
+__device__ __forceinline__ int full_grid_block(int NG, int grp_lo,
+                                               int grp_cnt) {
+  const int ngrp = (NG + kWave - 1) / kWave;
+  const int x = blockIdx.x;
+  return (x / grp_cnt) * ngrp + grp_lo + x % grp_cnt;
+}
+
 template <int K, bool MASK, bool N1LOG>
 __global__ __launch_bounds__(256) void unocc_fp64_kernel(
-    DevProblem p, int n1_begin, int n1_end, double *__restrict__ partials,
-    double *__restrict__ n1log) {
-  unocc_fp64_block<K, MASK, N1LOG>(p, n1_begin, n1_end, blockIdx.x,
-                                   partials + (size_t)blockIdx.x * kSlots,
-                                   n1log);
+    DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
+    double *__restrict__ partials, double *__restrict__ n1log) {
+  const int b = full_grid_block(p.NG, grp_lo, grp_cnt);
+  unocc_fp64_block<K, MASK, N1LOG>(p, n1_begin, n1_end, b,
+                                   partials + (size_t)b * kSlots, n1log);
 }
 
 // ---------------------------------------------------------------------------
@@ -545,12 +557,13 @@ __device__ __forceinline__ void unocc_mixed_block(
 
 template <int K, bool N1LOG>
 __global__ __launch_bounds__(256) void unocc_mixed_kernel(
-    DevProblem p, int n1_begin, int n1_end, double *__restrict__ partials,
-    double *__restrict__ n1log, float thresh,
+    DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
+    double *__restrict__ partials, double *__restrict__ n1log, float thresh,
     unsigned long long *__restrict__ fallback_count) {
-  unocc_mixed_block<K, N1LOG>(p, n1_begin, n1_end, blockIdx.x,
-                              partials + (size_t)blockIdx.x * kSlots, n1log,
-                              thresh, fallback_count);
+  const int b = full_grid_block(p.NG, grp_lo, grp_cnt);
+  unocc_mixed_block<K, N1LOG>(p, n1_begin, n1_end, b,
+                              partials + (size_t)b * kSlots, n1log, thresh,
+                              fallback_count);
 }
 
 // ---------------------------------------------------------------------------
@@ -855,13 +868,9 @@ __global__ __launch_bounds__(256) void batch_kernel(
   }
 }
 
-__global__ void batch_reset_kernel(int *counters,
-                                   unsigned long long *fallback_count) {
-  if (threadIdx.x == 0) {
-    counters[0] = 0;
-    counters[1] = 0;
-    *fallback_count = 0ull;
-  }
+// Clears the fallback counter with a kernel store (capturable in a graph).
+__global__ void batch_reset_kernel(unsigned long long *fallback_count) {
+  if (threadIdx.x == 0) *fallback_count = 0ull;
 }
 
 // One block per problem: fixed-order sum of its records (as reduce_kernel).
@@ -940,43 +949,55 @@ static hipError_t launch_unocc_k(const DevProblem &p, int n1b, int n1e,
                                  int precision, int sem, int waves,
                                  double thresh, double *partials,
                                  double *n1log, unsigned long long *fb,
-                                 int grid, hipStream_t s) {
+                                 int glo, int gcnt, int grid, hipStream_t s) {
   const dim3 blk(kWave * waves);
   const bool lg = n1log != nullptr;
   if (precision == 0) {
     if (sem == kSemBgw) {
       if (lg)
         unocc_fp64_kernel<K, true, true>
-            <<<grid, blk, 0, s>>>(p, n1b, n1e, partials, n1log);
+            <<<grid, blk, 0, s>>>(p, n1b, n1e, glo, gcnt, partials, n1log);
       else
         unocc_fp64_kernel<K, true, false>
-            <<<grid, blk, 0, s>>>(p, n1b, n1e, partials, n1log);
+            <<<grid, blk, 0, s>>>(p, n1b, n1e, glo, gcnt, partials, n1log);
     } else {
       if (lg)
         unocc_fp64_kernel<K, false, true>
-            <<<grid, blk, 0, s>>>(p, n1b, n1e, partials, n1log);
+            <<<grid, blk, 0, s>>>(p, n1b, n1e, glo, gcnt, partials, n1log);
       else
         unocc_fp64_kernel<K, false, false>
-            <<<grid, blk, 0, s>>>(p, n1b, n1e, partials, n1log);
+            <<<grid, blk, 0, s>>>(p, n1b, n1e, glo, gcnt, partials, n1log);
     }
   } else {
     if (sem == kSemBgw) return hipErrorNotSupported;
     if (lg)
       unocc_mixed_kernel<K, true><<<grid, blk, 0, s>>>(
-          p, n1b, n1e, partials, n1log, (float)thresh, fb);
+          p, n1b, n1e, glo, gcnt, partials, n1log, (float)thresh, fb);
     else
       unocc_mixed_kernel<K, false><<<grid, blk, 0, s>>>(
-          p, n1b, n1e, partials, n1log, (float)thresh, fb);
+          p, n1b, n1e, glo, gcnt, partials, n1log, (float)thresh, fb);
   }
   return hipGetLastError();
+}
+
+int unocc_groups(const DevProblem &p) {
+  return p.NG > 0 ? (p.NG + kWave - 1) / kWave : 0;
 }
 
 hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
                         int precision, int sem, const TileCfg &t,
                         double res_thresh, double *partials, double *n1log,
-                        unsigned long long *fallback_count, hipStream_t s) {
-  const int grid = unocc_grid(p, t);
-  if (grid == 0 || n1_begin >= n1_end) return hipSuccess;
+                        unsigned long long *fallback_count, hipStream_t s,
+                        int grp_lo, int grp_cnt) {
+  const int ngrp = unocc_groups(p);
+  if (grp_cnt < 0) {
+    grp_lo = 0;
+    grp_cnt = ngrp;
+  }
+  if (grp_lo < 0 || grp_lo + grp_cnt > ngrp) return hipErrorInvalidValue;
+  const int full = unocc_grid(p, t);
+  if (full == 0 || grp_cnt == 0 || n1_begin >= n1_end) return hipSuccess;
+  const int grid = full / ngrp * grp_cnt;
   // __launch_bounds__(256): at most 4 waves per block.
   if (t.waves < 1 || t.waves > 4) return hipErrorInvalidValue;
   switch (t.k_ig) {
@@ -984,7 +1005,7 @@ hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
   case KK:                                                                  \
     return launch_unocc_k<KK>(p, n1_begin, n1_end, precision, sem, t.waves, \
                               res_thresh, partials, n1log, fallback_count,  \
-                              grid, s);
+                              grp_lo, grp_cnt, grid, s);
     GPP_CASE(4)
     GPP_CASE(8)
     GPP_CASE(16)
@@ -1066,42 +1087,6 @@ hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
 
 // ---- batch
 
-template <int K, int PREC, int SEM, int KIND>
-static const void *batch_fn() {
-  return reinterpret_cast<const void *>(&batch_kernel<K, PREC, SEM, KIND>);
-}
-
-static const void *batch_entry(int k, int precision, int sem, int kind) {
-  if (kind == 1) {  // occupied items: FP64, tile-independent
-    if (sem == kSemGppkerStrict) return batch_fn<4, 0, kSemGppkerStrict, 1>();
-    if (sem == kSemBgw) return batch_fn<4, 0, kSemBgw, 1>();
-    return batch_fn<4, 0, kSemGppker, 1>();
-  }
-#define GPP_B(KK)                                                            \
-  if (k == KK) {                                                             \
-    if (precision == 1) return batch_fn<KK, 1, kSemGppker, 0>();             \
-    if (sem == kSemGppkerStrict) return batch_fn<KK, 0, kSemGppkerStrict, 0>(); \
-    if (sem == kSemBgw) return batch_fn<KK, 0, kSemBgw, 0>();                \
-    return batch_fn<KK, 0, kSemGppker, 0>();                                 \
-  }
-  GPP_B(4)
-  GPP_B(8)
-  GPP_B(16)
-  GPP_B(32)
-#undef GPP_B
-  return nullptr;
-}
-
-int batch_blocks_per_cu(int k_ig, int precision, int sem, int kind) {
-  const void *f = batch_entry(k_ig, precision, sem, kind);
-  if (!f) return 0;
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 256, 0) !=
-      hipSuccess)
-    return 1;
-  return nb > 0 ? nb : 1;
-}
-
 hipError_t launch_reduce_batch(const double *partials, const int *rec_begin,
                                const int *rec_count, int nprob, double *out,
                                hipStream_t s) {
@@ -1114,16 +1099,14 @@ hipError_t launch_reduce_batch(const double *partials, const int *rec_begin,
 hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
                         int n_u, const BatchItem *items_o, int n_o, int k_ig,
                         int precision, int sem, double res_thresh,
-                        double *partials, int *counters,
-                        unsigned long long *fallback_count, int unocc_threads,
-                        int grid_o, const int *rec_begin,
+                        double *partials, unsigned long long *fallback_count,
+                        int unocc_threads, const int *rec_begin,
                         const int *rec_count, int nprob, double *out,
                         hipStream_t s) {
-  (void)grid_o;
   if (unocc_threads < 64 || unocc_threads > 256 || unocc_threads % 64)
     return hipErrorInvalidValue;
   if (precision == 1 && sem == kSemBgw) return hipErrorNotSupported;
-  batch_reset_kernel<<<1, 64, 0, s>>>(counters, fallback_count);
+  batch_reset_kernel<<<1, 64, 0, s>>>(fallback_count);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const float thr = (float)res_thresh;

@@ -60,12 +60,18 @@ hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, size_t n,
                                hipStream_t s);
 hipError_t launch_gather_aqsm(const double2 *aqsm, const int *igp_of, int NB,
                               int NC, int NG, double2 *out, hipStream_t s);
+// Blocks of the non-occupied launch over all G' columns (= records).
 int unocc_grid(const DevProblem &p, const TileCfg &t);
+// G' groups of 64 columns (a launch may cover a contiguous range of them).
+int unocc_groups(const DevProblem &p);
 // n1log: NULL, or [grid][n1_end - n1_begin] complex per-band records.
+// grp_cnt < 0: all G' groups; else groups [grp_lo, grp_lo + grp_cnt), whose
+// records land where the launch over all groups puts them.
 hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
                         int precision, int sem, const TileCfg &t,
                         double res_thresh, double *partials, double *n1log,
-                        unsigned long long *fallback_count, hipStream_t s);
+                        unsigned long long *fallback_count, hipStream_t s,
+                        int grp_lo = 0, int grp_cnt = -1);
 int occ_grid(const DevProblem &p);
 // n1log: NULL, or [grid][n1_end] complex per-band records.
 hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
@@ -78,20 +84,18 @@ hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,
 int achstemp_records(const DevProblem &p, int sem);
 hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
                            hipStream_t s);
-// Batched launch followed by one reduction block per problem into
-// out[prob][12].  kind 0 = non-occupied items, 1 = occupied items.
-int batch_blocks_per_cu(int k_ig, int precision, int sem, int kind);
+// One reduction block per problem: records [rec_begin, +rec_count) ->
+// out[prob][12].
 hipError_t launch_reduce_batch(const double *partials, const int *rec_begin,
                                const int *rec_count, int nprob, double *out,
                                hipStream_t s);
 // Two flat launches (non-occupied items, then occupied items), one block per
-// item of each list; counters[0..1] are cleared by the reset kernel.
+// item of each list, then launch_reduce_batch into out.
 hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
                         int n_u, const BatchItem *items_o, int n_o, int k_ig,
                         int precision, int sem, double res_thresh,
-                        double *partials, int *counters,
-                        unsigned long long *fallback_count, int unocc_threads,
-                        int grid_o, const int *rec_begin,
+                        double *partials, unsigned long long *fallback_count,
+                        int unocc_threads, const int *rec_begin,
                         const int *rec_count, int nprob, double *out,
                         hipStream_t s);
 

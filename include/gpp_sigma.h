@@ -63,7 +63,11 @@ typedef struct { double re, im; } gpp_c128;
                                     cutoff for wx < 0, asxtemp -= ...      */
 
 /* Where the problem arrays live. */
-#define GPP_MEM_HOST     0   /* host pointers; the library stages them     */
+#define GPP_MEM_HOST     0   /* host pointers (pageable is fine); copied
+                                through a pinned ring by host threads, and
+                                I_eps/wtilde in chunks of G' rows whose
+                                kernel launches overlap the next chunk's
+                                copy; same bits as device inputs         */
 #define GPP_MEM_DEVICE   1   /* device pointers on the handle's device     */
 
 /* gpp_options.flags */
@@ -119,7 +123,9 @@ typedef struct {
                                      the caller, NULL to skip
                                      (gppKer.cpp:357,370-373)             */
   double unocc_kernel_ms;         /* n1 >= nvband kernel, HIP events on the
-                                     stream it ran on                      */
+                                     stream it ran on (host inputs: its
+                                     chunked launches and the copies they
+                                     wait for)                             */
   double occ_kernel_ms;           /* n1 < nvband kernel                    */
   double total_ms;                /* whole device section incl. reduction  */
   double terms;                   /* NB * NG * NC * 3 evaluated            */

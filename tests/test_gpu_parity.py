@@ -193,6 +193,27 @@ def test_device_memory_path(handle):
     check(got, orc.sigma(inp))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_host_pipeline_matches_device(handle, precision):
+    """Host inputs go over in chunks of G' rows, one non-occupied launch per
+    chunk; the chunks' blocks write the records the single launch writes, so
+    host and device inputs give the same bits.  NG = 2000: 32 groups of 64
+    columns (the last one partial), 11 chunks."""
+    import torch
+    say("host pipeline", precision)
+    inp = gppk.fill_inputs(3, 5, 8000, 4, nvband=2, seed=91)
+    dev = {k: (torch.from_numpy(np.ascontiguousarray(v)).cuda() if isinstance(v, np.ndarray) else v)
+           for k, v in inp.items()}
+    torch.cuda.synchronize()
+    a = handle.sigma(inp, precision=precision, acht_n1_loc=True)
+    b = handle.sigma(dev, precision=precision, memory=gppk.MEM_DEVICE, acht_n1_loc=True)
+    for k in ("achtemp", "asxtemp", "acht_n1_loc"):
+        assert np.array_equal(a[k], b[k]), (k, a[k], b[k])
+    assert a["achstemp"] == b["achstemp"]
+    if precision == "fp64":
+        check(a, orc.sigma(inp))
+
+
 def test_canonical_shape_band_subset(handle):
     """512/2/32768/20 inputs; the oracle runs the first 3 bands (2 occupied,
     1 unoccupied) at full ncouls and ngpown; the GPU runs the same slice and
