@@ -10,11 +10,11 @@
 //     for all 64 lanes (scalar loads, SGPR operands); conj(aqsm[n1][igp])
 //     differs per lane but is constant over the ig run, so it multiplies
 //     the run's partial sum once per band instead of once per term;
-//   * the three frequencies share one reciprocal: 1/(d0 d1 d2) by
-//     v_rcp_f64 plus one Newton step, then six multiplies give 1/d_iw
-//     (batch inversion).  A zero/denormal/non-finite product falls back to
-//     three plain divisions so that IEEE inf/NaN behave as in the
-//     reference;
+//   * the three frequencies of two G-vectors share one reciprocal: the
+//     product of the six |wdiff|^2 by v_rcp_f64 plus one Newton step, then
+//     multiplies give each 1/|wdiff|^2 (batch inversion).  A zero/denormal/
+//     non-finite product falls back to plain divisions so that IEEE
+//     inf/NaN behave as in the reference;
 //   * lane partials are weighted by 0.5*vcoul[igp], reduced across the wave
 //     with cross-lane shuffles, across the block through LDS, and written
 //     to one record per block; a second kernel sums the records in a fixed
@@ -176,7 +176,11 @@ __global__ void to_fp32_kernel(const double2 *__restrict__ in,
 // Equals conj(aqsm)*aqsn*delw*I_eps of gppKer.cpp:328-337 term by term.
 // n1log (optional): [block][n1 - n1_begin] complex, the block's share of
 // acht_n1_loc (gppKer.cpp:357).
-template <int K, bool MASK, bool N1LOG>
+// DIRECT: the problem arrives in kernel arguments (unocc_fp64_kernel).  The
+// batched launch reads it from memory per block; there the compiler keeps
+// the band-ahead aqsn run in VGPRs instead of SGPRs and the pair inversion
+// needs more than 168 VGPRs, so that variant uses neither.
+template <int K, bool MASK, bool N1LOG, bool DIRECT>
 __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
                                                  int n1_begin, int n1_end,
                                                  int b, double *__restrict__ rec,
@@ -230,7 +234,7 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
   // SGPRs) are loaded a whole band ahead, so no scalar-load latency sits
   // between the bands; for larger K (or the masked variant's extra live
   // values) the registers do not hold two runs.
-  constexpr bool kAhead = K <= 8 && !MASK && !N1LOG;
+  constexpr bool kAhead = DIRECT && K <= 8 && !MASK && !N1LOG;
   double2 an_cur[K];
   double x_cur[3] = {0.0, 0.0, 0.0};
   if (kAhead && n1_begin < n1_end) {
@@ -266,43 +270,85 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
     double s_r0 = 0.0, s_i0 = 0.0, s_r1 = 0.0, s_i1 = 0.0, s_r2 = 0.0,
            s_i2 = 0.0;
     bool bad = false;
+    // G-vectors in groups of G = DIRECT ? 2 : 1.  The 3G values |wdiff|^2 of a
+    // group share one reciprocal (batch inversion: a product tree, one
+    // v_rcp_f64 + Newton step, then back-substitution).  Pairs need 15
+    // multiplies for six reciprocals instead of 12 for two groups of three,
+    // but one v_rcp_f64 (a quarter-rate instruction) instead of two.
+    constexpr bool PAIR = DIRECT;
+    constexpr int G = PAIR ? 2 : 1;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double2 an = kAhead ? an_cur[k] : an_row[min(ig0 + k, NC - 1)];
-      const double Br = an.x * Ar[k] - an.y * Ai[k];
-      const double Bi = an.x * Ai[k] + an.y * Ar[k];
-      const double nBiwi = -Bi * wi[k];
-      const double Brwi = Br * wi[k];
-      const double dr0 = x0 - wr[k];
-      const double dr1 = x1 - wr[k];
-      const double dr2 = x2 - wr[k];
-      const double d0 = fma(dr0, dr0, wi2[k]);
-      const double d1 = fma(dr1, dr1, wi2[k]);
-      const double d2 = fma(dr2, dr2, wi2[k]);
-      const double p01 = d0 * d1;
-      const double pp = p01 * d2;
-      bad |= !is_pos_normal(pp);
-      double q = rcp_f64(pp);
-      const double e = fma(-pp, q, 1.0);
-      q = fma(q, e, q);
-      double r2 = q * p01;
-      const double t = q * d2;
-      double r0 = t * d1;
-      double r1 = t * d0;
-      if (MASK) {
-        // gppkernel.deslippe.09.30.2014.f90:364-366 /
-        // BGW-Kernels/gppKer.cpp:406: keep the term only if
-        // |wdiff|^2 > limittwo and |delw|^2 = |wtilde|^2/|wdiff|^2 < limitone.
-        r0 = (d0 > kLimitTwo && wt2[k] * r0 < kLimitOne) ? r0 : 0.0;
-        r1 = (d1 > kLimitTwo && wt2[k] * r1 < kLimitOne) ? r1 : 0.0;
-        r2 = (d2 > kLimitTwo && wt2[k] * r2 < kLimitOne) ? r2 : 0.0;
+    for (int k = 0; k < K; k += G) {
+      double Br[G], Bi[G], nBiwi[G], Brwi[G], dr[G][3], d[G][3], r[G][3];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const double2 an =
+            kAhead ? an_cur[k + j] : an_row[min(ig0 + k + j, NC - 1)];
+        Br[j] = an.x * Ar[k + j] - an.y * Ai[k + j];
+        Bi[j] = an.x * Ai[k + j] + an.y * Ar[k + j];
+        nBiwi[j] = -Bi[j] * wi[k + j];
+        Brwi[j] = Br[j] * wi[k + j];
+        dr[j][0] = x0 - wr[k + j];
+        dr[j][1] = x1 - wr[k + j];
+        dr[j][2] = x2 - wr[k + j];
+#pragma unroll
+        for (int iw = 0; iw < 3; ++iw)
+          d[j][iw] = fma(dr[j][iw], dr[j][iw], wi2[k + j]);
       }
-      s_r0 = fma(fma(Br, dr0, nBiwi), r0, s_r0);
-      s_i0 = fma(fma(Bi, dr0, Brwi), r0, s_i0);
-      s_r1 = fma(fma(Br, dr1, nBiwi), r1, s_r1);
-      s_i1 = fma(fma(Bi, dr1, Brwi), r1, s_i1);
-      s_r2 = fma(fma(Br, dr2, nBiwi), r2, s_r2);
-      s_i2 = fma(fma(Bi, dr2, Brwi), r2, s_i2);
+      if constexpr (PAIR) {
+        // Tree (depth 3 each way) rather than a running product, so the
+        // dependent chain around the reciprocal stays short.
+        const double a = d[0][0] * d[0][1];
+        const double b = d[0][2] * d[1][0];
+        const double c = d[1][1] * d[1][2];
+        const double ab = a * b;
+        const double pp = ab * c;
+        bad |= !is_pos_normal(pp);
+        double q = rcp_f64(pp);
+        const double e = fma(-pp, q, 1.0);
+        q = fma(q, e, q);           // 1 / (a b c)
+        const double qab = q * c;   // 1 / (a b)
+        const double qc = q * ab;   // 1 / c
+        const double qa = qab * b;  // 1 / a
+        const double qb = qab * a;  // 1 / b
+        r[0][0] = qa * d[0][1];
+        r[0][1] = qa * d[0][0];
+        r[0][2] = qb * d[1][0];
+        r[1][0] = qb * d[0][2];
+        r[1][1] = qc * d[1][2];
+        r[1][2] = qc * d[1][1];
+      } else {
+        const double p01 = d[0][0] * d[0][1];
+        const double pp = p01 * d[0][2];
+        bad |= !is_pos_normal(pp);
+        double q = rcp_f64(pp);
+        const double e = fma(-pp, q, 1.0);
+        q = fma(q, e, q);          // 1 / (d0 d1 d2)
+        const double t = q * d[0][2];  // 1 / (d0 d1)
+        r[0][2] = q * p01;
+        r[0][0] = t * d[0][1];
+        r[0][1] = t * d[0][0];
+      }
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        if (MASK) {
+          // gppkernel.deslippe.09.30.2014.f90:364-366 /
+          // BGW-Kernels/gppKer.cpp:406: keep the term only if |wdiff|^2 >
+          // limittwo and |delw|^2 = |wtilde|^2/|wdiff|^2 < limitone.
+#pragma unroll
+          for (int iw = 0; iw < 3; ++iw)
+            r[j][iw] = (d[j][iw] > kLimitTwo &&
+                        wt2[k + j] * r[j][iw] < kLimitOne)
+                           ? r[j][iw]
+                           : 0.0;
+        }
+        s_r0 = fma(fma(Br[j], dr[j][0], nBiwi[j]), r[j][0], s_r0);
+        s_i0 = fma(fma(Bi[j], dr[j][0], Brwi[j]), r[j][0], s_i0);
+        s_r1 = fma(fma(Br[j], dr[j][1], nBiwi[j]), r[j][1], s_r1);
+        s_i1 = fma(fma(Bi[j], dr[j][1], Brwi[j]), r[j][1], s_i1);
+        s_r2 = fma(fma(Br[j], dr[j][2], nBiwi[j]), r[j][2], s_r2);
+        s_i2 = fma(fma(Bi[j], dr[j][2], Brwi[j]), r[j][2], s_i2);
+      }
     }
     if (__builtin_expect(bad, 0)) {
       // Some product d0*d1*d2 was zero, denormal, infinite or NaN: redo this
@@ -385,12 +431,17 @@ __device__ __forceinline__ int full_grid_block(int NG, int grp_lo,
   return (x / grp_cnt) * ngrp + grp_lo + x % grp_cnt;
 }
 
+// For K <= 8 the register budget is capped at 3 waves per SIMD (168 VGPRs):
+// left alone the compiler takes a few more for scheduling and falls to 2
+// waves; the cap costs only spills outside the band loop (or one reload per
+// band in the masked variant).
 template <int K, bool MASK, bool N1LOG>
-__global__ __launch_bounds__(256) void unocc_fp64_kernel(
+__global__ __launch_bounds__(256) __attribute__((
+    amdgpu_waves_per_eu(K <= 8 ? 3 : 1))) void unocc_fp64_kernel(
     DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
     double *__restrict__ partials, double *__restrict__ n1log) {
   const int b = full_grid_block(p.NG, grp_lo, grp_cnt);
-  unocc_fp64_block<K, MASK, N1LOG>(p, n1_begin, n1_end, b,
+  unocc_fp64_block<K, MASK, N1LOG, true>(p, n1_begin, n1_end, b,
                                    partials + (size_t)b * kSlots, n1log);
 }
 
@@ -847,7 +898,9 @@ __global__ __launch_bounds__(kAchThreads) void achstemp_bgw_kernel(
 // atomic counter was measured too: the loop made the compiler keep 250
 // VGPRs instead of 164, one wave per SIMD less, and it ran slower.)
 // Non-occupied and occupied items are two launches (KIND 0 / 1) so each
-// keeps its own register budget.
+// keeps its own register budget.  The FP64 items invert |wdiff|^2 per
+// G-vector (not per pair): with the problem table read per block, the pair
+// form needs more than 168 VGPRs (2 waves per SIMD) and ran slower.
 template <int K, int PREC, int SEM, int KIND>
 __global__ __launch_bounds__(256) void batch_kernel(
     const DevProblem *__restrict__ probs, const BatchItem *__restrict__ items,
@@ -858,8 +911,8 @@ __global__ __launch_bounds__(256) void batch_kernel(
   double *rec = partials + (size_t)w.rec * kSlots;
   if (KIND == 0) {
     if (PREC == 0)
-      unocc_fp64_block<K, SEM == kSemBgw, false>(p, p.NV, p.NB, w.block, rec,
-                                                 nullptr);
+      unocc_fp64_block<K, SEM == kSemBgw, false, false>(p, p.NV, p.NB,
+                                                        w.block, rec, nullptr);
     else
       unocc_mixed_block<K, false>(p, p.NV, p.NB, w.block, rec, nullptr,
                                   thresh, fallback_count);
