@@ -38,6 +38,22 @@ __device__ __forceinline__ float rcp_f32(float x) {
 __device__ __forceinline__ bool is_pos_normal(double x) {
   return __builtin_amdgcn_class(x, 0x100);
 }
+// Loads of complex elements through generic pointers (kernel-argument
+// problems) or global-address-space pointers (problems read from memory by
+// the batched launch, where that type lets uniform loads become scalar).
+template <typename T>
+using GlobalPtr = __attribute__((address_space(1))) const T *;
+__device__ __forceinline__ double2 ld(const double2 *q) { return *q; }
+__device__ __forceinline__ float2 ld(const float2 *q) { return *q; }
+__device__ __forceinline__ double2 ld(GlobalPtr<double2> q) {
+  const GlobalPtr<double> d = (GlobalPtr<double>)q;
+  return make_double2(d[0], d[1]);
+}
+__device__ __forceinline__ float2 ld(GlobalPtr<float2> q) {
+  const GlobalPtr<float> f = (GlobalPtr<float>)q;
+  return make_float2(f[0], f[1]);
+}
+
 __device__ __forceinline__ int uniform(int x) {
   return __builtin_amdgcn_readfirstlane(x);
 }
@@ -176,12 +192,8 @@ __global__ void to_fp32_kernel(const double2 *__restrict__ in,
 // Equals conj(aqsm)*aqsn*delw*I_eps of gppKer.cpp:328-337 term by term.
 // n1log (optional): [block][n1 - n1_begin] complex, the block's share of
 // acht_n1_loc (gppKer.cpp:357).
-// DIRECT: the problem arrives in kernel arguments (unocc_fp64_kernel).  The
-// batched launch reads it from memory per block; there the compiler keeps
-// the band-ahead aqsn run in VGPRs instead of SGPRs and the pair inversion
-// needs more than 168 VGPRs, so that variant uses neither.
-template <int K, bool MASK, bool N1LOG, bool DIRECT>
-__device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
+template <int K, bool MASK, bool N1LOG, typename Prob>
+__device__ __forceinline__ void unocc_fp64_block(const Prob &p,
                                                  int n1_begin, int n1_end,
                                                  int b, double *__restrict__ rec,
                                                  double *__restrict__ n1log) {
@@ -205,8 +217,8 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
     const int ig = ig0 + k;
     if (valid && ig < NC) {
       const size_t off = (size_t)my_igp * NC + ig;
-      const double2 w = p.wtilde[off];
-      const double2 e = p.ieps[off];
+      const double2 w = ld(p.wtilde + off);
+      const double2 e = ld(p.ieps + off);
       wr[k] = w.x;
       wi[k] = w.y;
       Ar[k] = w.x * e.x - w.y * e.y;
@@ -226,21 +238,23 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
   const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double acc_r[3] = {0.0, 0.0, 0.0}, acc_i[3] = {0.0, 0.0, 0.0};
   // aqsm gathered per band into [n1][my_igp] (coalesced across lanes).
-  const double2 *aqsm_col = p.aqsmG + min(my_igp, NG - 1);
-  double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NG]
-                                        : make_double2(0.0, 0.0);
+  const auto aqsm_col = p.aqsmG + min(my_igp, NG - 1);
+  double2 am_next = (n1_begin < n1_end)
+                        ? ld(aqsm_col + (size_t)n1_begin * NG)
+                        : make_double2(0.0, 0.0);
 
   // For K <= 8 the next band's aqsn run and frequencies (wave-uniform, in
   // SGPRs) are loaded a whole band ahead, so no scalar-load latency sits
   // between the bands; for larger K (or the masked variant's extra live
   // values) the registers do not hold two runs.
-  constexpr bool kAhead = DIRECT && K <= 8 && !MASK && !N1LOG;
+  constexpr bool kAhead = K <= 8 && !MASK && !N1LOG;
   double2 an_cur[K];
   double x_cur[3] = {0.0, 0.0, 0.0};
   if (kAhead && n1_begin < n1_end) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      an_cur[k] = p.aqsn[(size_t)n1_begin * NC + min(ig0 + k, NC - 1)];
+      an_cur[k] =
+          ld(p.aqsn + (size_t)n1_begin * NC + min(ig0 + k, NC - 1));
 #pragma unroll
     for (int iw = 0; iw < 3; ++iw) x_cur[iw] = p.wx[n1_begin * 3 + iw];
   }
@@ -249,13 +263,13 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
     const double2 am = am_next;
     const int n1n = min(n1 + 1, n1_end - 1);
     // Prefetch next band's aqsm (index clamped: no branch in the loop).
-    am_next = aqsm_col[(size_t)n1n * NG];
+    am_next = ld(aqsm_col + (size_t)n1n * NG);
     double2 an_nxt[K];
     double x_nxt[3];
     if (kAhead) {
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        an_nxt[k] = p.aqsn[(size_t)n1n * NC + min(ig0 + k, NC - 1)];
+        an_nxt[k] = ld(p.aqsn + (size_t)n1n * NC + min(ig0 + k, NC - 1));
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) x_nxt[iw] = p.wx[n1n * 3 + iw];
     } else {
@@ -265,25 +279,25 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
     const double x0 = x_cur[0];
     const double x1 = x_cur[1];
     const double x2 = x_cur[2];
-    const double2 *an_row = p.aqsn + (size_t)n1 * NC;
+    const auto an_row = p.aqsn + (size_t)n1 * NC;
 
     double s_r0 = 0.0, s_i0 = 0.0, s_r1 = 0.0, s_i1 = 0.0, s_r2 = 0.0,
            s_i2 = 0.0;
     bool bad = false;
-    // G-vectors in groups of G = DIRECT ? 2 : 1.  The 3G values |wdiff|^2 of a
-    // group share one reciprocal (batch inversion: a product tree, one
-    // v_rcp_f64 + Newton step, then back-substitution).  Pairs need 15
-    // multiplies for six reciprocals instead of 12 for two groups of three,
-    // but one v_rcp_f64 (a quarter-rate instruction) instead of two.
-    constexpr bool PAIR = DIRECT;
-    constexpr int G = PAIR ? 2 : 1;
+    // G-vectors in pairs: the six |wdiff|^2 of a pair share one reciprocal
+    // (batch inversion: a product tree, one v_rcp_f64 + Newton step, then
+    // back-substitution).  That takes 15 multiplies for six reciprocals
+    // instead of 12 for two groups of three, but one v_rcp_f64 (a
+    // quarter-rate instruction) instead of two.
+    constexpr int G = 2;
+    static_assert(K % G == 0, "K must be even");
 #pragma unroll
     for (int k = 0; k < K; k += G) {
       double Br[G], Bi[G], nBiwi[G], Brwi[G], dr[G][3], d[G][3], r[G][3];
 #pragma unroll
       for (int j = 0; j < G; ++j) {
         const double2 an =
-            kAhead ? an_cur[k + j] : an_row[min(ig0 + k + j, NC - 1)];
+            kAhead ? an_cur[k + j] : ld(an_row + min(ig0 + k + j, NC - 1));
         Br[j] = an.x * Ar[k + j] - an.y * Ai[k + j];
         Bi[j] = an.x * Ai[k + j] + an.y * Ar[k + j];
         nBiwi[j] = -Bi[j] * wi[k + j];
@@ -295,40 +309,27 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
         for (int iw = 0; iw < 3; ++iw)
           d[j][iw] = fma(dr[j][iw], dr[j][iw], wi2[k + j]);
       }
-      if constexpr (PAIR) {
-        // Tree (depth 3 each way) rather than a running product, so the
-        // dependent chain around the reciprocal stays short.
-        const double a = d[0][0] * d[0][1];
-        const double b = d[0][2] * d[1][0];
-        const double c = d[1][1] * d[1][2];
-        const double ab = a * b;
-        const double pp = ab * c;
-        bad |= !is_pos_normal(pp);
-        double q = rcp_f64(pp);
-        const double e = fma(-pp, q, 1.0);
-        q = fma(q, e, q);           // 1 / (a b c)
-        const double qab = q * c;   // 1 / (a b)
-        const double qc = q * ab;   // 1 / c
-        const double qa = qab * b;  // 1 / a
-        const double qb = qab * a;  // 1 / b
-        r[0][0] = qa * d[0][1];
-        r[0][1] = qa * d[0][0];
-        r[0][2] = qb * d[1][0];
-        r[1][0] = qb * d[0][2];
-        r[1][1] = qc * d[1][2];
-        r[1][2] = qc * d[1][1];
-      } else {
-        const double p01 = d[0][0] * d[0][1];
-        const double pp = p01 * d[0][2];
-        bad |= !is_pos_normal(pp);
-        double q = rcp_f64(pp);
-        const double e = fma(-pp, q, 1.0);
-        q = fma(q, e, q);          // 1 / (d0 d1 d2)
-        const double t = q * d[0][2];  // 1 / (d0 d1)
-        r[0][2] = q * p01;
-        r[0][0] = t * d[0][1];
-        r[0][1] = t * d[0][0];
-      }
+      // Tree (depth 3 each way) rather than a running product, so the
+      // dependent chain around the reciprocal stays short.
+      const double a = d[0][0] * d[0][1];
+      const double b = d[0][2] * d[1][0];
+      const double c = d[1][1] * d[1][2];
+      const double ab = a * b;
+      const double pp = ab * c;
+      bad |= !is_pos_normal(pp);
+      double q = rcp_f64(pp);
+      const double e = fma(-pp, q, 1.0);
+      q = fma(q, e, q);           // 1 / (a b c)
+      const double qab = q * c;   // 1 / (a b)
+      const double qc = q * ab;   // 1 / c
+      const double qa = qab * b;  // 1 / a
+      const double qb = qab * a;  // 1 / b
+      r[0][0] = qa * d[0][1];
+      r[0][1] = qa * d[0][0];
+      r[0][2] = qb * d[1][0];
+      r[1][0] = qb * d[0][2];
+      r[1][1] = qc * d[1][2];
+      r[1][2] = qc * d[1][1];
 #pragma unroll
       for (int j = 0; j < G; ++j) {
         if (MASK) {
@@ -351,9 +352,9 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
       }
     }
     if (__builtin_expect(bad, 0)) {
-      // Some product d0*d1*d2 was zero, denormal, infinite or NaN: redo this
-      // lane's run with one division per term so that exact resonances give
-      // the reference's inf/NaN in their own frequency only.
+      // Some pair's product of |wdiff|^2 was zero, denormal, infinite or
+      // NaN: redo this lane's run with one division per term so that exact
+      // resonances give the reference's inf/NaN in their own frequency only.
       // The operands are reloaded (this path is rare) and the loop is kept
       // rolled, so that it adds no register pressure to the hot loop.
       s_r0 = s_i0 = s_r1 = s_i1 = s_r2 = s_i2 = 0.0;
@@ -362,11 +363,11 @@ __device__ __forceinline__ void unocc_fp64_block(const DevProblem &p,
         const int ig = ig0 + k;
         if (!valid || ig >= NC) continue;  // inert slot: contributes 0
         const size_t off = (size_t)my_igp * NC + ig;
-        const double2 w = p.wtilde[off];
-        const double2 e = p.ieps[off];
+        const double2 w = ld(p.wtilde + off);
+        const double2 e = ld(p.ieps + off);
         const double A_r = w.x * e.x - w.y * e.y;
         const double A_i = w.x * e.y + w.y * e.x;
-        const double2 an = an_row[ig];
+        const double2 an = ld(an_row + ig);
         const double Br = an.x * A_r - an.y * A_i;
         const double Bi = an.x * A_i + an.y * A_r;
         const double wy2 = w.y * w.y;
@@ -441,7 +442,7 @@ __global__ __launch_bounds__(256) __attribute__((
     DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
     double *__restrict__ partials, double *__restrict__ n1log) {
   const int b = full_grid_block(p.NG, grp_lo, grp_cnt);
-  unocc_fp64_block<K, MASK, N1LOG, true>(p, n1_begin, n1_end, b,
+  unocc_fp64_block<K, MASK, N1LOG>(p, n1_begin, n1_end, b,
                                    partials + (size_t)b * kSlots, n1log);
 }
 
@@ -456,9 +457,9 @@ __global__ __launch_bounds__(256) __attribute__((
 // are FP64.  If any term of a lane's run has d < thresh (near or exact
 // resonance, FP32 underflow), that lane's whole run for this band is redone
 // in FP64 from the FP64 arrays and replaces the FP32 sum.
-template <int K, bool N1LOG>
+template <int K, bool N1LOG, typename Prob>
 __device__ __forceinline__ void unocc_mixed_block(
-    const DevProblem &p, int n1_begin, int n1_end, int b,
+    const Prob &p, int n1_begin, int n1_end, int b,
     double *__restrict__ rec, double *__restrict__ n1log, float thresh,
     unsigned long long *__restrict__ fallback_count) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -480,8 +481,8 @@ __device__ __forceinline__ void unocc_mixed_block(
     const int ig = ig0 + k;
     if (valid && ig < NC) {
       const size_t off = (size_t)my_igp * NC + ig;
-      const double2 w = p.wtilde[off];
-      const double2 e = p.ieps[off];
+      const double2 w = ld(p.wtilde + off);
+      const double2 e = ld(p.ieps + off);
       wrh[k] = (float)w.x;
       wrl[k] = (float)(w.x - (double)wrh[k]);
       wi[k] = (float)w.y;
@@ -502,13 +503,14 @@ __device__ __forceinline__ void unocc_mixed_block(
   const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
   double acc_r[3] = {0.0, 0.0, 0.0}, acc_i[3] = {0.0, 0.0, 0.0};
   unsigned long long nfb = 0;
-  const double2 *aqsm_col = p.aqsmG + min(my_igp, NG - 1);
-  double2 am_next = (n1_begin < n1_end) ? aqsm_col[(size_t)n1_begin * NG]
-                                        : make_double2(0.0, 0.0);
+  const auto aqsm_col = p.aqsmG + min(my_igp, NG - 1);
+  double2 am_next = (n1_begin < n1_end)
+                        ? ld(aqsm_col + (size_t)n1_begin * NG)
+                        : make_double2(0.0, 0.0);
 
   for (int n1 = n1_begin; n1 < n1_end; ++n1) {
     const double2 am = am_next;
-    am_next = aqsm_col[(size_t)min(n1 + 1, n1_end - 1) * NG];
+    am_next = ld(aqsm_col + (size_t)min(n1 + 1, n1_end - 1) * NG);
     const double x[3] = {p.wx[n1 * 3 + 0], p.wx[n1 * 3 + 1],
                          p.wx[n1 * 3 + 2]};
     float xh[3], xl[3];
@@ -517,14 +519,14 @@ __device__ __forceinline__ void unocc_mixed_block(
       xh[iw] = (float)x[iw];
       xl[iw] = (float)(x[iw] - (double)xh[iw]);
     }
-    const float2 *an_row = p.aqsn32 + (size_t)n1 * NC;
+    const auto an_row = p.aqsn32 + (size_t)n1 * NC;
 
     float s_r[3] = {0.f, 0.f, 0.f}, s_i[3] = {0.f, 0.f, 0.f};
     float dmin = 3.0e38f;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int igk = min(ig0 + k, NC - 1);
-      const float2 an = an_row[igk];
+      const float2 an = ld(an_row + igk);
       const float Br = an.x * Ar[k] - an.y * Ai[k];
       const float Bi = an.x * Ai[k] + an.y * Ar[k];
       const float nBiwi = -Bi * wi[k];
@@ -550,15 +552,15 @@ __device__ __forceinline__ void unocc_mixed_block(
       // Rare: the lane's run is redone in FP64 (rolled loop, reloads, so
       // that it adds no register pressure to the loop above).
       sr[0] = sr[1] = sr[2] = si[0] = si[1] = si[2] = 0.0;
-      const double2 *an64 = p.aqsn + (size_t)n1 * NC;
+      const auto an64 = p.aqsn + (size_t)n1 * NC;
 #pragma unroll 1
       for (int k = 0; k < K; ++k) {
         const int ig = ig0 + k;
         if (!valid || ig >= NC) continue;
         const size_t off = (size_t)my_igp * NC + ig;
-        const double2 w = p.wtilde[off];
-        const double2 e = p.ieps[off];
-        const double2 a = an64[ig];
+        const double2 w = ld(p.wtilde + off);
+        const double2 e = ld(p.ieps + off);
+        const double2 a = ld(an64 + ig);
         const double A_r = w.x * e.x - w.y * e.y;
         const double A_i = w.x * e.y + w.y * e.x;
         const double B_r = a.x * A_r - a.y * A_i;
@@ -626,8 +628,8 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
 constexpr int kOccThreads = 256;
 constexpr int kOccChunk = 2048;  // G-vectors per block
 
-template <int SEM, bool N1LOG>
-__device__ __forceinline__ void occ_block(const DevProblem &p, int n1_end,
+template <int SEM, bool N1LOG, typename Prob>
+__device__ __forceinline__ void occ_block(const Prob &p, int n1_end,
                                           int b, double *__restrict__ rec,
                                           double *__restrict__ n1log) {
   const int NC = p.NC, NG = p.NG;
@@ -649,25 +651,25 @@ __device__ __forceinline__ void occ_block(const DevProblem &p, int n1_end,
   double ssx_r[3] = {0, 0, 0}, ssx_i[3] = {0, 0, 0};
   const int ig_end = min(NC, (ch + 1) * kOccChunk);
   for (int n1 = 0; n1 < n1_end; ++n1) {
-    const double2 am = p.aqsm[(size_t)n1 * NC + igp];
+    const double2 am = ld(p.aqsm + (size_t)n1 * NC + igp);
     double tsr[3] = {0, 0, 0}, tsi[3] = {0, 0, 0};
     double txr[3] = {0, 0, 0}, txi[3] = {0, 0, 0};
     for (int ig = ch * kOccChunk + threadIdx.x; ig < ig_end;
          ig += blockDim.x) {
       const size_t off = (size_t)my_igp * NC + ig;
-      const double2 w = p.wtilde[off];
-      const double2 e = p.ieps[off];
+      const double2 w = ld(p.wtilde + off);
+      const double2 e = ld(p.ieps + off);
       // I_eps used for sch and for the cutoff.  gppKer.cpp:139 passes
       // I_eps_array[my_igp*ngpown+ig]; the strict mode keeps that index.
       const double2 ex =
-          (SEM == kSemGppkerStrict) ? p.ieps[(size_t)my_igp * NG + ig] : e;
+          (SEM == kSemGppkerStrict) ? ld(p.ieps + (size_t)my_igp * NG + ig) : e;
       // wtilde2 = wtilde^2, Omega2 = wtilde2 * I_eps (gppKer.cpp:132-133)
       const double w2r = w.x * w.x - w.y * w.y;
       const double w2i = 2.0 * w.x * w.y;
       const double omr = w2r * e.x - w2i * e.y;
       const double omi = w2r * e.y + w2i * e.x;
       const double cut2 = kSexcut * kSexcut * (ex.x * ex.x + ex.y * ex.y);
-      const double2 an = p.aqsn[(size_t)n1 * NC + ig];
+      const double2 an = ld(p.aqsn + (size_t)n1 * NC + ig);
       // matngmatmgp = aqsn[n1][ig] * conj(aqsm[n1][igp])
       const double mr = an.x * am.x + an.y * am.y;
       const double mi = an.y * am.x - an.x * am.y;
@@ -890,6 +892,35 @@ __global__ __launch_bounds__(kAchThreads) void achstemp_bgw_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// DevProblem with its arrays typed as global memory (address space 1).
+struct DevProblemG {
+  GlobalPtr<double2> aqsm, aqsn, ieps, wtilde;
+  GlobalPtr<double> vcoul;
+  GlobalPtr<int> igp_of;
+  GlobalPtr<double> wx;
+  GlobalPtr<float2> aqsn32;
+  GlobalPtr<double2> aqsmG;
+  int NB, NV, NC, NG;
+};
+
+__device__ __forceinline__ DevProblemG global_view(const DevProblem &p) {
+  DevProblemG g;
+  g.aqsm = (GlobalPtr<double2>)p.aqsm;
+  g.aqsn = (GlobalPtr<double2>)p.aqsn;
+  g.ieps = (GlobalPtr<double2>)p.ieps;
+  g.wtilde = (GlobalPtr<double2>)p.wtilde;
+  g.vcoul = (GlobalPtr<double>)p.vcoul;
+  g.igp_of = (GlobalPtr<int>)p.igp_of;
+  g.wx = (GlobalPtr<double>)p.wx;
+  g.aqsn32 = (GlobalPtr<float2>)p.aqsn32;
+  g.aqsmG = (GlobalPtr<double2>)p.aqsmG;
+  g.NB = p.NB;
+  g.NV = p.NV;
+  g.NC = p.NC;
+  g.NG = p.NG;
+  return g;
+}
+
 // Batched launch: one grid over a work list of (problem, block) items of
 // all problems, sorted largest-first on the host, so the hardware
 // dispatcher hands the longest blocks out first and the short ones fill the
@@ -898,21 +929,24 @@ __global__ __launch_bounds__(kAchThreads) void achstemp_bgw_kernel(
 // atomic counter was measured too: the loop made the compiler keep 250
 // VGPRs instead of 164, one wave per SIMD less, and it ran slower.)
 // Non-occupied and occupied items are two launches (KIND 0 / 1) so each
-// keeps its own register budget.  The FP64 items invert |wdiff|^2 per
-// G-vector (not per pair): with the problem table read per block, the pair
-// form needs more than 168 VGPRs (2 waves per SIMD) and ran slower.
+// keeps its own register budget (capped at 3 waves per SIMD for K <= 8, as
+// for unocc_fp64_kernel).
 template <int K, int PREC, int SEM, int KIND>
-__global__ __launch_bounds__(256) void batch_kernel(
+__global__ __launch_bounds__(256) __attribute__((
+    amdgpu_waves_per_eu(K <= 8 ? 3 : 1))) void batch_kernel(
     const DevProblem *__restrict__ probs, const BatchItem *__restrict__ items,
     double *__restrict__ partials, float thresh,
     unsigned long long *__restrict__ fallback_count) {
   const BatchItem w = items[blockIdx.x];
-  const DevProblem p = probs[w.prob];
+  // Pointers read from memory are generic (flat) to the compiler; the
+  // global-address-space view lets uniform loads become scalar loads, as
+  // they are for the kernel-argument problem of the direct launches.
+  const DevProblemG p = global_view(probs[w.prob]);
   double *rec = partials + (size_t)w.rec * kSlots;
   if (KIND == 0) {
     if (PREC == 0)
-      unocc_fp64_block<K, SEM == kSemBgw, false, false>(p, p.NV, p.NB,
-                                                        w.block, rec, nullptr);
+      unocc_fp64_block<K, SEM == kSemBgw, false>(p, p.NV, p.NB, w.block, rec,
+                                                 nullptr);
     else
       unocc_mixed_block<K, false>(p, p.NV, p.NB, w.block, rec, nullptr,
                                   thresh, fallback_count);
