@@ -54,6 +54,34 @@ __device__ __forceinline__ float2 ld(GlobalPtr<float2> q) {
   return make_float2(f[0], f[1]);
 }
 
+// 1/x: v_rcp_f64 plus one Newton step (about 1e-15 relative) for positive
+// normal x, IEEE division otherwise (zero, denormal, inf, NaN), so special
+// values come out as the reference's division gives them.
+__device__ __forceinline__ double recip(double x) {
+  if (__builtin_expect(!is_pos_normal(x), 0)) return 1.0 / x;
+  const double q = rcp_f64(x);
+  return fma(q, fma(-x, q, 1.0), q);
+}
+
+// r[i] = 1/d[i] for three values with one v_rcp_f64 (batch inversion);
+// IEEE divisions if the product is not a positive normal number.
+__device__ __forceinline__ void recip3(double d0, double d1, double d2,
+                                       double &r0, double &r1, double &r2) {
+  const double p01 = d0 * d1;
+  const double pp = p01 * d2;
+  double q = rcp_f64(pp);
+  q = fma(q, fma(-pp, q, 1.0), q);  // 1 / (d0 d1 d2)
+  const double t = q * d2;          // 1 / (d0 d1)
+  r2 = q * p01;
+  r0 = t * d1;
+  r1 = t * d0;
+  if (__builtin_expect(!is_pos_normal(pp), 0)) {
+    r0 = 1.0 / d0;
+    r1 = 1.0 / d1;
+    r2 = 1.0 / d2;
+  }
+}
+
 __device__ __forceinline__ int uniform(int x) {
   return __builtin_amdgcn_readfirstlane(x);
 }
@@ -624,15 +652,132 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
 // gppKer.cpp:16-62,122-146 (SEM gppker, gppker-strict) or the occupied
 // branch of gppkernel.deslippe.09.30.2014.f90:271-334 (SEM bgw).
 // Block b: my_igp = b / nch, G-vector chunk b % nch; threads stride over the
-// chunk.  Bands outer so that the per-band log needs no per-band registers.
+// chunk.  Each thread reads wtilde and I_eps of its G-vector once and runs
+// the occupied bands inside, so those rows cross HBM once, not once per band
+// (the per-band log variant keeps bands outer: it needs one block sum per
+// band).
 constexpr int kOccThreads = 256;
 constexpr int kOccChunk = 2048;  // G-vectors per block
+
+// The band-independent part of one (my_igp, ig) of the occupied solver.
+struct OccG {
+  double wr, wi, wy2;  // wtilde, wtilde.im^2
+  double w2r, w2i;     // wtilde2 = wtilde^2 (gppKer.cpp:132)
+  double omr, omi;     // Omega2 = wtilde2 * I_eps (gppKer.cpp:133)
+  double exr, exi;     // the I_eps entry used for sch and the cutoff
+  double cut2;         // (sexcut * |I_eps|)^2
+};
+
+template <int SEM, typename Prob>
+__device__ __forceinline__ OccG occ_g(const Prob &p, int my_igp, int ig) {
+  const int NC = p.NC, NG = p.NG;
+  const size_t off = (size_t)my_igp * NC + ig;
+  const double2 w = ld(p.wtilde + off);
+  const double2 e = ld(p.ieps + off);
+  // gppKer.cpp:139 passes I_eps_array[my_igp*ngpown+ig] to the solver; the
+  // strict mode keeps that index, the others read [my_igp][ig].
+  const double2 ex =
+      (SEM == kSemGppkerStrict) ? ld(p.ieps + (size_t)my_igp * NG + ig) : e;
+  OccG g;
+  g.wr = w.x;
+  g.wi = w.y;
+  g.wy2 = w.y * w.y;
+  g.w2r = w.x * w.x - g.wy2;
+  g.w2i = 2.0 * w.x * w.y;
+  g.omr = g.w2r * e.x - g.w2i * e.y;
+  g.omi = g.w2r * e.y + g.w2i * e.x;
+  g.exr = ex.x;
+  g.exi = ex.y;
+  g.cut2 = kSexcut * kSexcut * (ex.x * ex.x + ex.y * ex.y);
+  return g;
+}
+
+// The band's frequencies and, for the gppker first branch, the factor
+// cden / |cden|^2 with cden = wxt^2 (real, the same for every G-vector).
+struct OccBand {
+  double wxt[3], fssx[3];
+};
+
+template <typename Prob>
+__device__ __forceinline__ OccBand occ_band(const Prob &p, int n1) {
+  OccBand B;
+#pragma unroll
+  for (int iw = 0; iw < 3; ++iw) {
+    B.wxt[iw] = p.wx[n1 * 3 + iw];
+    const double c = B.wxt[iw] * B.wxt[iw];
+    B.fssx[iw] = c * recip(c * c);
+  }
+  return B;
+}
+
+// One (n1, my_igp, ig), three frequencies: adds matngmatmgp * sch to ts and
+// matngmatmgp * ssx to tx (gppKer.cpp:27-61, 141-142).
+template <int SEM>
+__device__ __forceinline__ void occ_term(const OccG &g, const OccBand &B,
+                                         double mr, double mi,
+                                         double (&tsr)[3], double (&tsi)[3],
+                                         double (&txr)[3], double (&txi)[3]) {
+  // wdiff = wxt - wtilde = (dr, -wi); conj(wdiff) = (dr, wi)
+  double dr[3], wdiffr[3], rden[3];
+#pragma unroll
+  for (int iw = 0; iw < 3; ++iw) {
+    dr[iw] = B.wxt[iw] - g.wr;
+    wdiffr[iw] = fma(dr[iw], dr[iw], g.wy2);
+  }
+  recip3(wdiffr[0], wdiffr[1], wdiffr[2], rden[0], rden[1], rden[2]);
+#pragma unroll
+  for (int iw = 0; iw < 3; ++iw) {
+    // delw = wtilde * conj(wdiff) * rden
+    const double dlr = (g.wr * dr[iw] - g.wy2) * rden[iw];
+    const double dli = (g.wr * g.wi + g.wi * dr[iw]) * rden[iw];
+    const double delwr = dlr * dlr + dli * dli;
+    double schr = 0.0, schi = 0.0, ssr = 0.0, ssi = 0.0;
+    if (wdiffr[iw] > kLimitTwo && delwr < kLimitOne) {
+      schr = dlr * g.exr - dli * g.exi;
+      schi = dlr * g.exi + dli * g.exr;
+      if (SEM == kSemBgw) {
+        // cden = wxt^2 - wtilde2 (f90:303)
+        const double cr = B.wxt[iw] * B.wxt[iw] - g.w2r;
+        const double ci = -g.w2i;
+        const double rd = recip(cr * cr + ci * ci);
+        ssr = (g.omr * cr + g.omi * ci) * rd;
+        ssi = (g.omi * cr - g.omr * ci) * rd;
+      } else {
+        ssr = g.omr * B.fssx[iw];
+        ssi = g.omi * B.fssx[iw];
+      }
+    } else if (delwr > kTo1) {
+      // cden = 4 * wtilde2 * (delw + 0.5)
+      const double ar = dlr + 0.5, ai = dli;
+      const double cr = 4.0 * (g.w2r * ar - g.w2i * ai);
+      const double ci = 4.0 * (g.w2r * ai + g.w2i * ar);
+      const double rd = recip(cr * cr + ci * ci);
+      // ssx = -Omega2 * conj(cden) * rden * delw
+      const double tr = -(g.omr * cr + g.omi * ci) * rd;
+      const double ti = -(g.omi * cr - g.omr * ci) * rd;
+      ssr = tr * dlr - ti * dli;
+      ssi = tr * dli + ti * dlr;
+    }
+    if (SEM == kSemBgw) {
+      // f90:321: |ssx| > sexcut*|I_eps| and wxt < 0 -> ssx = 0
+      if (B.wxt[iw] < 0.0 && (ssr * ssr + ssi * ssi) > g.cut2) {
+        ssr = 0.0;
+        ssi = 0.0;
+      }
+    }
+    // (gppKer.cpp:58 tests abs(wxt) < 0, which never holds.)
+    txr[iw] += mr * ssr - mi * ssi;
+    txi[iw] += mr * ssi + mi * ssr;
+    tsr[iw] += mr * schr - mi * schi;
+    tsi[iw] += mr * schi + mi * schr;
+  }
+}
 
 template <int SEM, bool N1LOG, typename Prob>
 __device__ __forceinline__ void occ_block(const Prob &p, int n1_end,
                                           int b, double *__restrict__ rec,
                                           double *__restrict__ n1log) {
-  const int NC = p.NC, NG = p.NG;
+  const int NC = p.NC;
   const int nch = (NC + kOccChunk - 1) / kOccChunk;
   const int my_igp = b / nch;
   const int ch = b % nch;
@@ -647,104 +792,52 @@ __device__ __forceinline__ void occ_block(const Prob &p, int n1_end,
   const double vc = p.vcoul[igp];
   const double wsch = 0.5 * vc;                              // :310,355
   const double wssx = (SEM == kSemBgw ? -kOcc : kOcc) * vc;  // :352 / f90:393
-  double sch_r[3] = {0, 0, 0}, sch_i[3] = {0, 0, 0};
-  double ssx_r[3] = {0, 0, 0}, ssx_i[3] = {0, 0, 0};
+  double tsr[3] = {0, 0, 0}, tsi[3] = {0, 0, 0};
+  double txr[3] = {0, 0, 0}, txi[3] = {0, 0, 0};
+  const int ig_begin = ch * kOccChunk + threadIdx.x;
   const int ig_end = min(NC, (ch + 1) * kOccChunk);
-  for (int n1 = 0; n1 < n1_end; ++n1) {
-    const double2 am = ld(p.aqsm + (size_t)n1 * NC + igp);
-    double tsr[3] = {0, 0, 0}, tsi[3] = {0, 0, 0};
-    double txr[3] = {0, 0, 0}, txi[3] = {0, 0, 0};
-    for (int ig = ch * kOccChunk + threadIdx.x; ig < ig_end;
-         ig += blockDim.x) {
-      const size_t off = (size_t)my_igp * NC + ig;
-      const double2 w = ld(p.wtilde + off);
-      const double2 e = ld(p.ieps + off);
-      // I_eps used for sch and for the cutoff.  gppKer.cpp:139 passes
-      // I_eps_array[my_igp*ngpown+ig]; the strict mode keeps that index.
-      const double2 ex =
-          (SEM == kSemGppkerStrict) ? ld(p.ieps + (size_t)my_igp * NG + ig) : e;
-      // wtilde2 = wtilde^2, Omega2 = wtilde2 * I_eps (gppKer.cpp:132-133)
-      const double w2r = w.x * w.x - w.y * w.y;
-      const double w2i = 2.0 * w.x * w.y;
-      const double omr = w2r * e.x - w2i * e.y;
-      const double omi = w2r * e.y + w2i * e.x;
-      const double cut2 = kSexcut * kSexcut * (ex.x * ex.x + ex.y * ex.y);
-      const double2 an = ld(p.aqsn + (size_t)n1 * NC + ig);
-      // matngmatmgp = aqsn[n1][ig] * conj(aqsm[n1][igp])
-      const double mr = an.x * am.x + an.y * am.y;
-      const double mi = an.y * am.x - an.x * am.y;
+  if (N1LOG) {
+    for (int n1 = 0; n1 < n1_end; ++n1) {
+      const double2 am = ld(p.aqsm + (size_t)n1 * NC + igp);
+      const OccBand B = occ_band(p, n1);
+      double bsr[3] = {0, 0, 0}, bsi[3] = {0, 0, 0};
+      for (int ig = ig_begin; ig < ig_end; ig += blockDim.x) {
+        const OccG g = occ_g<SEM>(p, my_igp, ig);
+        const double2 an = ld(p.aqsn + (size_t)n1 * NC + ig);
+        // matngmatmgp = aqsn[n1][ig] * conj(aqsm[n1][igp])
+        const double mr = an.x * am.x + an.y * am.y;
+        const double mi = an.y * am.x - an.x * am.y;
+        occ_term<SEM>(g, B, mr, mi, bsr, bsi, txr, txi);
+      }
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) {
-        const double wxt = p.wx[n1 * 3 + iw];
-        // wdiff = wxt - wtilde = (dr, -w.y); conj(wdiff) = (dr, w.y)
-        const double dr = wxt - w.x;
-        const double wdiffr = dr * dr + w.y * w.y;
-        const double rden = 1.0 / wdiffr;
-        const double dlr = (w.x * dr - w.y * w.y) * rden;
-        const double dli = (w.x * w.y + w.y * dr) * rden;
-        const double delwr = dlr * dlr + dli * dli;
-        double schr = 0.0, schi = 0.0, ssr = 0.0, ssi = 0.0;
-        if (wdiffr > kLimitTwo && delwr < kLimitOne) {
-          schr = dlr * ex.x - dli * ex.y;
-          schi = dlr * ex.y + dli * ex.x;
-          if (SEM == kSemBgw) {
-            // cden = wxt^2 - wtilde2 (f90:303)
-            const double cr = wxt * wxt - w2r;
-            const double ci = -w2i;
-            const double rd = 1.0 / (cr * cr + ci * ci);
-            ssr = (omr * cr + omi * ci) * rd;
-            ssi = (omi * cr - omr * ci) * rd;
-          } else {
-            // cden = wxt^2 (real), ssx = Omega2 * conj(cden) / |cden|^2
-            const double c = wxt * wxt;
-            const double rd = 1.0 / (c * c);
-            ssr = omr * c * rd;
-            ssi = omi * c * rd;
-          }
-        } else if (delwr > kTo1) {
-          // cden = 4 * wtilde2 * (delw + 0.5)
-          const double ar = dlr + 0.5, ai = dli;
-          const double cr = 4.0 * (w2r * ar - w2i * ai);
-          const double ci = 4.0 * (w2r * ai + w2i * ar);
-          const double rd = 1.0 / (cr * cr + ci * ci);
-          // ssx = -Omega2 * conj(cden) * rden * delw
-          const double tr = -(omr * cr + omi * ci) * rd;
-          const double ti = -(omi * cr - omr * ci) * rd;
-          ssr = tr * dlr - ti * dli;
-          ssi = tr * dli + ti * dlr;
-        }
-        if (SEM == kSemBgw) {
-          // f90:321: |ssx| > sexcut*|I_eps| and wxt < 0 -> ssx = 0
-          if (wxt < 0.0 && (ssr * ssr + ssi * ssi) > cut2) {
-            ssr = 0.0;
-            ssi = 0.0;
-          }
-        }
-        // (gppKer.cpp:58 tests abs(wxt) < 0, which never holds.)
-        txr[iw] += mr * ssr - mi * ssi;
-        txi[iw] += mr * ssi + mi * ssr;
-        tsr[iw] += mr * schr - mi * schi;
-        tsi[iw] += mr * schi + mi * schr;
+        tsr[iw] += bsr[iw];
+        tsi[iw] += bsi[iw];
+      }
+      // acht_n1_loc[n1] += sch_array[2] * vcoul[igp]
+      n1log_store(wsch * bsr[2], wsch * bsi[2], n1log,
+                  (size_t)b * n1_end + n1);
+    }
+  } else {
+    for (int ig = ig_begin; ig < ig_end; ig += blockDim.x) {
+      const OccG g = occ_g<SEM>(p, my_igp, ig);
+      for (int n1 = 0; n1 < n1_end; ++n1) {
+        const double2 am = ld(p.aqsm + (size_t)n1 * NC + igp);
+        const OccBand B = occ_band(p, n1);
+        const double2 an = ld(p.aqsn + (size_t)n1 * NC + ig);
+        const double mr = an.x * am.x + an.y * am.y;
+        const double mi = an.y * am.x - an.x * am.y;
+        occ_term<SEM>(g, B, mr, mi, tsr, tsi, txr, txi);
       }
     }
-#pragma unroll
-    for (int iw = 0; iw < 3; ++iw) {
-      sch_r[iw] += tsr[iw];
-      sch_i[iw] += tsi[iw];
-      ssx_r[iw] += txr[iw];
-      ssx_i[iw] += txi[iw];
-    }
-    if (N1LOG)  // acht_n1_loc[n1] += sch_array[2] * vcoul[igp]
-      n1log_store(wsch * tsr[2], wsch * tsi[2], n1log,
-                  (size_t)b * n1_end + n1);
   }
   double v[12];
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {
-    v[2 * iw] = wsch * sch_r[iw];
-    v[2 * iw + 1] = wsch * sch_i[iw];
-    v[6 + 2 * iw] = wssx * ssx_r[iw];
-    v[6 + 2 * iw + 1] = wssx * ssx_i[iw];
+    v[2 * iw] = wsch * tsr[iw];
+    v[2 * iw + 1] = wsch * tsi[iw];
+    v[6 + 2 * iw] = wssx * txr[iw];
+    v[6 + 2 * iw + 1] = wssx * txi[iw];
   }
   block_store<12>(v, rec);
 }

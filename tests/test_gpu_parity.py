@@ -107,6 +107,31 @@ def test_exact_resonance_nan_pattern(handle):
     check(g2, orc.sigma(q, "bgw"))
 
 
+@pytest.mark.parametrize("sem", ["gppker", "strict", "bgw"])
+def test_occupied_branches_and_exact_resonance(handle, sem):
+    """Occupied bands on band-resolved inputs whose wtilde sit near the
+    frequencies (both solver branches taken), plus two wtilde exactly at a
+    frequency.  In an occupied band (n1 = 0, iw = 2) |wdiff|^2 = 0 makes delw
+    NaN and the reference takes neither solver branch, so asxtemp stays
+    finite; in an unoccupied band (n1 = 5, iw = 1) achtemp[1] becomes NaN."""
+    say("occupied branches", sem)
+    inp = gppk.fill_inputs(3, 6, 900, 6, nvband=4, seed=17)
+    check(handle.sigma(inp, semantics=sem), orc.sigma(inp, sem))
+    w = inp["wtilde_array"].copy()
+    w[2, 9] = complex(inp["wx_array"][0, 2], 0.0)
+    w[1, 7] = complex(inp["wx_array"][5, 1], 0.0)
+    q = dict(inp, wtilde_array=w)
+    got = handle.sigma(q, semantics=sem)
+    ref = orc.sigma(q, sem)
+    assert np.all(np.isfinite(ref["asxtemp"]))
+    check(got, ref, keys=("asxtemp",))
+    if sem == "bgw":  # the masked non-occupied sum drops |wdiff|^2 <= 0.25
+        check(got, ref, keys=("achtemp",))
+    else:
+        assert np.isnan(ref["achtemp"][1]) and np.isnan(got["achtemp"][1])
+        assert rel_err(got["achtemp"][[0, 2]], ref["achtemp"][[0, 2]]) <= FP64_TOL
+
+
 def test_check_finite_flag(handle):
     inp = gppk.fill_inputs(2, 2, 256, 4, nvband=0, seed=5)
     w = inp["wtilde_array"].copy()
