@@ -8,7 +8,8 @@ the loader raises, there is no fallback.
 
 Reference interface mirrored: the gppKer loop nest (gppKer.cpp:270-361) and
 the operator split tillNvband_kernel / achtemp_kernel
-(gppKer_gpuComplex.cpp:127,148).
+(gppKer_gpuComplex.cpp:127,148); for the full-frequency kernels
+(include/ff_sigma.h) the kernels of FFKer_cori/ffKernel.cpp:107-300.
 """
 from __future__ import annotations
 
@@ -21,6 +22,7 @@ import numpy as np
 PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libgpp.so"
 CLI_PATH = PKG_DIR / "bin" / "gpp"
+FF_CLI_PATH = PKG_DIR / "bin" / "ff"
 
 GPP_OK = 0
 GPP_EINVAL = -1
@@ -45,12 +47,25 @@ FLAG_AUTOTUNE = 16
 SEMANTICS = {"gppker": SEM_GPPKER, "strict": SEM_GPPKER_STRICT, "bgw": SEM_BGW}
 PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
 
-# Exported symbols, one per declaration of include/gpp_sigma.h.
+FF_SEM_CPP = 0
+FF_SEM_F90 = 1
+FF_SEMANTICS = {"cpp": FF_SEM_CPP, "f90": FF_SEM_F90}
+FF_PART_ACHS = 1
+FF_PART_ASX = 2
+FF_PART_ACH = 4
+FF_PART_COR = 8
+FF_OUTPUTS = ("asxDtemp", "achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp")
+
+# Exported symbols, one per declaration of include/gpp_sigma.h and
+# include/ff_sigma.h.
 EXPORTS = (
     "gpp_create", "gpp_destroy", "gpp_last_error", "gpp_default_options",
     "gpp_sigma_ex", "gpp_sigma_batch", "gpp_autotune", "gpp_sigma",
     "gpp_achtemp_kernel", "gpp_tillNvband_kernel", "gpp_fill_inputs",
     "gpp_fill_inputs_device", "gpp_version",
+    "ff_default_options", "ff_sigma_ex", "ff_achsDtemp_Kernel",
+    "ff_asxDtemp_Kernel", "ff_achDtemp_Kernel", "ff_achDtemp_cor_Kernel",
+    "ff_fill_inputs", "ff_fill_inputs_device",
 )
 
 
@@ -92,6 +107,53 @@ class Result(C.Structure):
 _lib = None
 
 
+class FFProblem(C.Structure):
+    _fields_ = [
+        ("number_bands", C.c_int), ("nvband", C.c_int), ("ncouls", C.c_int),
+        ("ngpown", C.c_int), ("nFreq", C.c_int), ("nfreqeval", C.c_int),
+        ("aqsmtemp", C.c_void_p), ("aqsntemp", C.c_void_p),
+        ("I_epsR_array", C.c_void_p), ("I_epsA_array", C.c_void_p),
+        ("vcoul", C.c_void_p), ("inv_igp_index", C.c_void_p),
+        ("indinv", C.c_void_p), ("ekq", C.c_void_p), ("dFreqGrid", C.c_void_p),
+        ("pref", C.c_void_p), ("dFreqBrd", C.c_void_p),
+        ("schDt_matrix", C.c_void_p),
+        ("freqevalmin", C.c_double), ("freqevalstep", C.c_double),
+        ("pref_zb", C.c_double), ("occ", C.c_double),
+    ]
+
+
+class FFOptions(C.Structure):
+    _fields_ = [("semantics", C.c_int), ("memory", C.c_int),
+                ("stream", C.c_void_p), ("parts", C.c_int)]
+
+
+class FFResult(C.Structure):
+    _fields_ = [
+        ("achsDtemp", c128), ("asxDtemp", C.c_void_p), ("achDtemp", C.c_void_p),
+        ("achDtemp_cor", C.c_void_p), ("achDtemp_corb", C.c_void_p),
+        ("ach2Dtemp", C.c_void_p), ("slice_kernel_ms", C.c_double),
+        ("freq_kernel_ms", C.c_double), ("total_ms", C.c_double),
+        ("slice_bytes", C.c_double), ("slices", C.c_longlong),
+        ("slice_terms", C.c_longlong), ("freq_terms", C.c_longlong),
+    ]
+
+
+FF_ARRAYS = ("aqsmtemp", "aqsntemp", "I_epsR_array", "I_epsA_array", "vcoul",
+             "inv_igp_index", "indinv", "ekq", "dFreqGrid", "pref", "dFreqBrd",
+             "schDt_matrix")
+
+
+def make_ff_problem(inp: dict) -> FFProblem:
+    """FFProblem from a dict of arrays (numpy host or torch device) with the
+    keys of FF_ARRAYS plus number_bands, nvband, ncouls, ngpown, nFreq,
+    nfreqeval, freqevalmin, freqevalstep, pref_zb, occ."""
+    return FFProblem(int(inp["number_bands"]), int(inp["nvband"]),
+                     int(inp["ncouls"]), int(inp["ngpown"]), int(inp["nFreq"]),
+                     int(inp["nfreqeval"]), *[_ptr(inp.get(k)) for k in FF_ARRAYS],
+                     float(inp["freqevalmin"]), float(inp["freqevalstep"]),
+                     float(inp["pref_zb"]), float(inp["occ"]))
+
+
 def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     """Load libgpp.so (built in-tree by __graft_entry__.build()).  Raises
     OSError when it is missing: there is no CPU fallback."""
@@ -122,6 +184,17 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     lib.gpp_fill_inputs_device.argtypes = ([ip, C.c_ulonglong, ip, ip, ip, ip]
                                            + [vp] * 9)
     lib.gpp_version.restype = C.c_char_p
+    lib.ff_default_options.argtypes = [C.POINTER(FFOptions)]
+    lib.ff_sigma_ex.argtypes = [vp, C.POINTER(FFProblem), C.POINTER(FFOptions),
+                                C.POINTER(FFResult)]
+    lib.ff_achsDtemp_Kernel.argtypes = [ip, ip, ip] + [vp] * 7
+    lib.ff_asxDtemp_Kernel.argtypes = ([ip] * 6 + [C.c_double] * 3 + [vp] * 10)
+    lib.ff_achDtemp_Kernel.argtypes = ([ip] * 6 + [C.c_double] * 2 + [vp, C.c_double]
+                                       + [vp] * 8)
+    lib.ff_achDtemp_cor_Kernel.argtypes = ([ip] * 6 + [C.c_double] * 2 + [vp] * 15)
+    fill = [ip, ip, C.c_ulonglong, ip, ip, ip, ip] + [vp] * 13
+    lib.ff_fill_inputs.argtypes = fill
+    lib.ff_fill_inputs_device.argtypes = fill + [vp]
     if path is None:
         _lib = lib
     return lib
@@ -252,6 +325,33 @@ class Handle:
             out["acht_n1_loc"] = n1[: prob.number_bands]
         return out
 
+    def ff(self, inp: dict, semantics="cpp", memory=None, stream=None,
+           parts=0) -> dict:
+        """Full-frequency outputs (ff_sigma_ex): achsDtemp, the five
+        [nfreqeval] arrays, and the kernel times and work counts."""
+        opt = FFOptions()
+        self.lib.ff_default_options(C.byref(opt))
+        opt.semantics = FF_SEMANTICS[semantics]
+        if memory is None:
+            memory = MEM_DEVICE if hasattr(inp["aqsmtemp"], "data_ptr") else MEM_HOST
+        opt.memory = memory
+        opt.stream = stream
+        opt.parts = parts
+        ne = int(inp["nfreqeval"])
+        outs = {k: np.zeros(ne, np.complex128) for k in FF_OUTPUTS}
+        res = FFResult()
+        for k in FF_OUTPUTS:
+            setattr(res, k, outs[k].ctypes.data)
+        prob = make_ff_problem(inp)
+        rc = self.lib.ff_sigma_ex(self.h, C.byref(prob), C.byref(opt), C.byref(res))
+        if rc != GPP_OK:
+            raise GppError(rc, self.last_error())
+        outs["achsDtemp"] = _cplx(res.achsDtemp)
+        for k in ("slice_kernel_ms", "freq_kernel_ms", "total_ms", "slice_bytes",
+                  "slices", "slice_terms", "freq_terms"):
+            outs[k] = getattr(res, k)
+        return outs
+
     def batch(self, inputs: list, precision="fp64", semantics="gppker",
               memory=None, flags=0, stream=None, resonance_threshold=0.0,
               tile_ig=0) -> list:
@@ -344,6 +444,68 @@ def fill_inputs_device(config: int, number_bands: int, ncouls: int,
         raise GppError(rc, "gpp_fill_inputs_device failed")
     d["inv_igp_index"] = d["inv_igp_index"][:ng]
     d.update(number_bands=nb, ncouls=nc, ngpown=ng, nvband=nvband)
+    return d
+
+
+def _ff_arrays(nb, nc, ng, nf, alloc):
+    return {
+        "aqsmtemp": alloc((nb, nc), "c"), "aqsntemp": alloc((nb, nc), "c"),
+        "I_epsR_array": alloc((nf, ng, nc), "c"),
+        "I_epsA_array": alloc((nf, ng, nc), "c"),
+        "vcoul": alloc((nc,), "d"), "inv_igp_index": alloc((max(ng, 1),), "i"),
+        "indinv": alloc((nc,), "i"), "ekq": alloc((max(nb, 1),), "d"),
+        "dFreqGrid": alloc((nf,), "d"), "pref": alloc((nf,), "d"),
+        "dFreqBrd": alloc((nf,), "c"), "schDt_matrix": alloc((max(nb, 1), nf), "c"),
+    }
+
+
+def ff_fill_inputs(config: int, number_bands: int, nvband: int, ncouls: int,
+                   ngpown: int, nFreq: int, nfreqeval: int, semantics="cpp",
+                   seed: int = 2024, lib: C.CDLL | None = None) -> dict:
+    """FF inputs (numpy, host) from the library's generator (ff_fill_inputs):
+    config 1 the mini-app's fills, 2 seeded random."""
+    lib = lib or load_library()
+    dt = {"c": np.complex128, "d": np.float64, "i": np.int32}
+    d = _ff_arrays(number_bands, ncouls, ngpown, nFreq,
+                   lambda shape, t: np.empty(shape, dt[t]))
+    scal = np.empty(4, np.float64)
+    rc = lib.ff_fill_inputs(config, FF_SEMANTICS[semantics], seed, number_bands,
+                            ncouls, ngpown, nFreq, *[_ptr(d[k]) for k in FF_ARRAYS],
+                            _ptr(scal))
+    if rc != GPP_OK:
+        raise GppError(rc, "ff_fill_inputs failed")
+    d["inv_igp_index"] = d["inv_igp_index"][:ngpown].copy()
+    d.update(number_bands=number_bands, nvband=nvband, ncouls=ncouls,
+             ngpown=ngpown, nFreq=nFreq, nfreqeval=nfreqeval,
+             freqevalmin=scal[0], freqevalstep=scal[1], pref_zb=scal[2],
+             occ=scal[3])
+    return d
+
+
+def ff_fill_inputs_device(config: int, number_bands: int, nvband: int,
+                          ncouls: int, ngpown: int, nFreq: int, nfreqeval: int,
+                          semantics="cpp", seed: int = 2024,
+                          lib: C.CDLL | None = None) -> dict:
+    """The same FF inputs generated in device memory (torch tensors on
+    cuda:0) by ff_fill_inputs_device."""
+    import torch
+    lib = lib or load_library()
+    dev = torch.device("cuda", 0)
+    dt = {"c": torch.complex128, "d": torch.float64, "i": torch.int32}
+    d = _ff_arrays(number_bands, ncouls, ngpown, nFreq,
+                   lambda shape, t: torch.empty(shape, dtype=dt[t], device=dev))
+    scal = np.empty(4, np.float64)
+    rc = lib.ff_fill_inputs_device(config, FF_SEMANTICS[semantics], seed,
+                                   number_bands, ncouls, ngpown, nFreq,
+                                   *[d[k].data_ptr() for k in FF_ARRAYS],
+                                   _ptr(scal), torch.cuda.current_stream().cuda_stream)
+    if rc != GPP_OK:
+        raise GppError(rc, "ff_fill_inputs_device failed")
+    d["inv_igp_index"] = d["inv_igp_index"][:ngpown]
+    d.update(number_bands=number_bands, nvband=nvband, ncouls=ncouls,
+             ngpown=ngpown, nFreq=nFreq, nfreqeval=nfreqeval,
+             freqevalmin=scal[0], freqevalstep=scal[1], pref_zb=scal[2],
+             occ=scal[3])
     return d
 
 

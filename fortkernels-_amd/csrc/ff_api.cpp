@@ -1,0 +1,682 @@
+// ff_api.cpp — the C ABI of include/ff_sigma.h: the full-frequency (FF)
+// self-energy terms of FFKer_cori on the GPU.
+//
+// Per call:
+//   1. plan on the host: for every (band, evaluation frequency) the grid
+//      search of the chosen variant (compute_fact, ffKernel.cpp:28-78, or
+//      ffKernel.f90:231-245,391-430) gives two frequency slices and their
+//      weights; the distinct (array, slice, band) triples are collected
+//      and packed into passes of up to 8 bands per slice;
+//   2. device: cG = vcoul * conj(aqsm) per (band, G'); one HBM pass per
+//      needed slice computes the per-slice sums T(band, slice); the
+//      frequency-grid terms run in their own kernel;
+//   3. host: the outputs are the weighted combinations of the T values
+//      (the interpolations are linear) plus the grid terms.
+// Host inputs are copied through the handle's pinned ring, and only the
+// I_eps slices the evaluation frequencies touch are copied.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/ff_sigma.h"
+#include "ff_internal.h"
+#include "gpp_handle.h"
+
+using gpp::Arena;
+using gpp::ensure;
+using gpp::ensure_host_out;
+using gpp::set_err;
+using gpp::ff::SliceJob;
+
+namespace {
+
+enum Kind { kR = 0, kA = 1 };
+
+// Interpolation between slices f and f+1 with weights f1, f2; f < 0: the
+// variant computes nothing for this (band, frequency).
+struct Fact {
+  int f = -1;
+  double f1 = 0.0, f2 = 0.0;
+};
+
+// compute_fact's search (ffKernel.cpp:30-35): the last k with
+// g[k] < x < g[k+1], else 0.
+int search_open(double x, int nF, const double *g) {
+  int f = 0;
+  for (int k = 0; k < nF - 1; ++k)
+    if (x > g[k] && x < g[k + 1]) f = k;
+  return f;
+}
+
+// The F90 search (ffKernel.f90:231-245): the last k with
+// g[k] <= x < g[k+1], else the last interval.  For a strictly increasing
+// grid at most one interval matches, so a binary search finds it.
+int search_half_open(double x, int nF, const double *g, bool sorted) {
+  if (sorted) {
+    if (!(x >= g[0] && x < g[nF - 1])) return nF - 2;
+    int lo = 0, hi = nF - 1;  // g[lo] <= x < g[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) / 2;
+      if (g[mid] <= x)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    return lo;
+  }
+  int f = -1;
+  for (int k = 0; k < nF - 1; ++k)
+    if (x >= g[k] && x < g[k + 1]) f = k;
+  return f < 0 ? nF - 2 : f;
+}
+
+Fact weights(int f, double x, double scale, const double *g) {
+  Fact r;
+  r.f = f;
+  const double d = g[f + 1] - g[f];
+  r.f1 = scale * (g[f + 1] - x) / d;
+  r.f2 = scale * (x - g[f]) / d;
+  return r;
+}
+
+// asx (loop 1) and cor (loop 2) weights of the two variants.
+Fact fact(int sem, int loop, bool occ, double wx, int nF, const double *g,
+          bool sorted) {
+  Fact r;
+  if (sem == FF_SEM_CPP) {
+    if (loop == 1) {  // called with flag_occ = 0 (ffKernel.cpp:146)
+      const bool pos = wx > 0.0;
+      int f = search_open(pos ? wx : -wx, nF, g);
+      if (f == 0) f = nF - 2;
+      if (pos) return weights(f, wx, 1.0, g);
+      r = weights(f, -wx, 1.0, g);  // (g[f+1] + wx)/d, (-g[f] - wx)/d
+      return r;
+    }
+    if (wx > 0.0) {
+      if (occ) return r;  // achDtemp_cor_Kernel calls nothing (:228)
+      return weights(search_open(wx, nF, g), wx, -0.5, g);  // no fallback
+    }
+    if (!occ) return r;
+    int f = search_open(-wx, nF, g);
+    if (f == 0) f = nF - 2;
+    return weights(f, -wx, 1.0, g);
+  }
+  if (loop == 1) {  // occupied bands only
+    if (!occ) return r;
+    const double x = wx >= 0.0 ? wx : -wx;
+    return weights(search_half_open(x, nF, g, sorted), x, 1.0, g);
+  }
+  if (wx >= 0.0)
+    return weights(search_half_open(wx, nF, g, sorted), wx, -0.5, g);
+  if (!occ) return r;
+  return weights(search_half_open(-wx, nF, g, sorted), -wx, 1.0, g);
+}
+
+struct Cplx {
+  double re = 0.0, im = 0.0;
+  void add(double r, double i) {
+    re += r;
+    im += i;
+  }
+};
+
+int validate(gpp_handle h, const ff_problem *p, const ff_options *o,
+             int parts) {
+  if (!p || !o) return set_err(h, GPP_EINVAL, "null problem or options");
+  if (p->number_bands < 0 || p->nvband < 0 || p->ncouls <= 0 ||
+      p->ngpown < 0 || p->nFreq < 2 || p->nfreqeval < 0)
+    return set_err(h, GPP_EINVAL,
+                   "bad shape: number_bands=%d nvband=%d ncouls=%d ngpown=%d "
+                   "nFreq=%d nfreqeval=%d (nFreq must be >= 2)",
+                   p->number_bands, p->nvband, p->ncouls, p->ngpown, p->nFreq,
+                   p->nfreqeval);
+  if (o->semantics != FF_SEM_CPP && o->semantics != FF_SEM_F90)
+    return set_err(h, GPP_EINVAL, "bad semantics %d", o->semantics);
+  if (o->memory != GPP_MEM_HOST && o->memory != GPP_MEM_DEVICE)
+    return set_err(h, GPP_EINVAL, "bad memory space %d", o->memory);
+  if (parts & ~(FF_PART_ACHS | FF_PART_ASX | FF_PART_ACH | FF_PART_COR))
+    return set_err(h, GPP_EINVAL, "bad parts mask %d", parts);
+  if (p->number_bands > 65535 || (long long)p->nFreq * p->number_bands > (1LL << 31))
+    return set_err(h, GPP_EINVAL, "number_bands or nFreq too large");
+  const bool nb = p->number_bands > 0, ng = p->ngpown > 0;
+  const bool sums = parts & (FF_PART_ACHS | FF_PART_ASX | FF_PART_COR);
+  if (sums && nb && ng &&
+      (!p->aqsmtemp || !p->aqsntemp || !p->I_epsR_array || !p->vcoul ||
+       !p->inv_igp_index || !p->indinv))
+    return set_err(h, GPP_EINVAL, "null input array");
+  if ((parts & FF_PART_COR) && nb && ng && !p->I_epsA_array)
+    return set_err(h, GPP_EINVAL, "null I_epsA_array");
+  if ((parts & FF_PART_ASX) && o->semantics == FF_SEM_F90 && nb && ng &&
+      !p->I_epsA_array)
+    return set_err(h, GPP_EINVAL, "null I_epsA_array");
+  if ((parts & (FF_PART_ASX | FF_PART_ACH | FF_PART_COR)) && nb &&
+      (!p->ekq || !p->dFreqGrid))
+    return set_err(h, GPP_EINVAL, "null ekq or dFreqGrid");
+  if ((parts & FF_PART_ACH) && nb &&
+      (!p->pref || !p->dFreqBrd || !p->schDt_matrix))
+    return set_err(h, GPP_EINVAL, "null pref, dFreqBrd or schDt_matrix");
+  return GPP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ff_default_options(ff_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->semantics = FF_SEM_CPP;
+  o->memory = GPP_MEM_HOST;
+}
+
+int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
+                ff_result *r) {
+  if (!h) return GPP_EINVAL;
+  if (!r) return set_err(h, GPP_EINVAL, "null result");
+  h->err.clear();
+  const int parts =
+      (o && o->parts) ? o->parts
+                      : (FF_PART_ACHS | FF_PART_ASX | FF_PART_ACH | FF_PART_COR);
+  int rc = validate(h, p, o, parts);
+  if (rc) return rc;
+  HIP_OR_RET(h, hipSetDevice(h->device));
+  const bool host = o->memory == GPP_MEM_HOST;
+  hipStream_t s = (!host && o->stream) ? (hipStream_t)o->stream : h->stream;
+  const int sem = o->semantics;
+  const int NB = p->number_bands, NV = std::min(p->nvband, NB);
+  const int NC = p->ncouls, NG = p->ngpown, nF = p->nFreq, NE = p->nfreqeval;
+  const size_t per_slice = (size_t)NG * NC;
+
+  // Host copies of the two small arrays the plan needs.
+  std::vector<double> ekq(NB), grid(nF);
+  const bool need_grid = parts & (FF_PART_ASX | FF_PART_ACH | FF_PART_COR);
+  if (need_grid && NB > 0) {
+    const hipMemcpyKind k = host ? hipMemcpyHostToHost : hipMemcpyDeviceToHost;
+    HIP_OR_RET(h, hipMemcpy(ekq.data(), p->ekq, NB * sizeof(double), k));
+    HIP_OR_RET(h, hipMemcpy(grid.data(), p->dFreqGrid, nF * sizeof(double), k));
+  }
+  bool sorted = true;
+  for (int k = 0; k + 1 < nF && need_grid; ++k)
+    sorted = sorted && grid[k] < grid[k + 1];
+
+  // ---- plan: which (kind, slice, band) sums are needed
+  // need[(kind*nF + f)*NB + band] -> job*8 + slot, or -1
+  std::vector<int> need((size_t)2 * nF * NB, -1);
+  auto mark = [&](int kind, int f, int band) {
+    need[((size_t)kind * nF + f) * NB + band] = 0;
+  };
+  const bool sums_possible = NB > 0 && NG > 0;
+  const int achs_slice = sem == FF_SEM_CPP ? 1 : 0;
+  if ((parts & FF_PART_ACHS) && sums_possible)
+    for (int b = 0; b < NB; ++b) mark(kR, achs_slice, b);
+  // asx: [NV][NE] weights (C++: the same for every iw)
+  std::vector<Fact> fasx((size_t)NV * NE);
+  std::vector<signed char> kasx((size_t)NV * NE, kR);
+  if ((parts & FF_PART_ASX) && sums_possible)
+    for (int b = 0; b < NV; ++b)
+      for (int iw = 0; iw < NE; ++iw) {
+        const double wx = sem == FF_SEM_CPP
+                              ? p->freqevalmin - ekq[b] + p->freqevalstep
+                              : p->freqevalmin - ekq[b] + iw * p->freqevalstep;
+        Fact fc = (sem == FF_SEM_CPP && iw > 0)
+                      ? fasx[(size_t)b * NE]
+                      : fact(sem, 1, true, wx, nF, grid.data(), sorted);
+        const int kind = (sem == FF_SEM_F90 && wx < 0.0) ? kA : kR;
+        fasx[(size_t)b * NE + iw] = fc;
+        kasx[(size_t)b * NE + iw] = (signed char)kind;
+        if (fc.f >= 0) {
+          mark(kind, fc.f, b);
+          mark(kind, fc.f + 1, b);
+        }
+      }
+  // cor: [NB][NE]
+  std::vector<Fact> fcor((size_t)NB * NE);
+  std::vector<signed char> pcor((size_t)NB * NE, 0);
+  if ((parts & FF_PART_COR) && sums_possible)
+    for (int b = 0; b < NB; ++b)
+      for (int iw = 0; iw < NE; ++iw) {
+        const double wx = sem == FF_SEM_CPP
+                              ? p->freqevalmin - ekq[b] + p->freqevalstep
+                              : p->freqevalmin - ekq[b] + iw * p->freqevalstep;
+        const bool occ = b < NV;
+        Fact fc = (sem == FF_SEM_CPP && iw > 0)
+                      ? fcor[(size_t)b * NE]
+                      : fact(sem, 2, occ, wx, nF, grid.data(), sorted);
+        fcor[(size_t)b * NE + iw] = fc;
+        pcor[(size_t)b * NE + iw] =
+            (signed char)(sem == FF_SEM_CPP ? wx > 0.0 : wx >= 0.0);
+        if (fc.f >= 0)
+          for (int kind = kR; kind <= kA; ++kind) {
+            mark(kind, fc.f, b);
+            mark(kind, fc.f + 1, b);
+          }
+      }
+
+  // ---- pack passes: up to 8 bands per (kind, slice)
+  std::vector<SliceJob> jobs;
+  std::vector<int> job_slice;  // index into the staged slices (host memory)
+  std::vector<std::pair<int, int>> slices;  // (kind, f)
+  for (int kind = kR; kind <= kA; ++kind)
+    for (int f = 0; f < nF; ++f) {
+      int *row = &need[((size_t)kind * nF + f) * NB];
+      bool any = false;
+      for (int b = 0; b < NB && !any; ++b) any = row[b] >= 0;
+      if (!any) continue;
+      const int si = (int)slices.size();
+      slices.push_back({kind, f});
+      SliceJob j{};
+      for (int b = 0; b < NB; ++b) {
+        if (row[b] < 0) continue;
+        if (j.nb == gpp::ff::kMaxJobBands) {
+          jobs.push_back(j);
+          job_slice.push_back(si);
+          j = SliceJob{};
+        }
+        row[b] = (int)jobs.size() * gpp::ff::kMaxJobBands + j.nb;
+        j.band[j.nb++] = b;
+      }
+      jobs.push_back(j);
+      job_slice.push_back(si);
+    }
+  const int njobs = (int)jobs.size();
+  if (njobs > 65535)
+    return set_err(h, GPP_EUNSUPPORTED, "%d slice passes (max 65535)", njobs);
+
+  // ---- device layout (h->ff) and staging
+  const int nchunk = gpp::ff::slice_chunks(NC);
+  int ngroups = 1;
+  if (njobs > 0) {
+    const long long per = (long long)njobs * nchunk;
+    ngroups = (int)std::min<long long>(
+        std::max(1, NG / 4), std::max<long long>(1, (2048 + per - 1) / per));
+  }
+  const int per_job = ngroups * nchunk;
+  const bool do_ach = (parts & FF_PART_ACH) && NB > 0 && NE > 0;
+  Arena a;
+  const size_t o_cg = a.take((size_t)NB * NG * 16);
+  const size_t o_jobs = a.take(jobs.size() * sizeof(SliceJob));
+  const size_t o_rec = a.take((size_t)njobs * per_job * gpp::ff::kRecSlots * 8);
+  const size_t o_T = a.take((size_t)njobs * gpp::ff::kRecSlots * 8);
+  const size_t o_part = a.take(do_ach ? (size_t)NB * NE * gpp::ff::kFreqSlots * 8 : 0);
+  const size_t o_fout = a.take(do_ach ? (size_t)NE * gpp::ff::kFreqSlots * 8 : 0);
+  // host inputs: small arrays and aqsm/aqsn staged here too
+  const bool need_aqs = njobs > 0;
+  size_t o_aqsm = 0, o_aqsn = 0, o_vc = 0, o_inv = 0, o_ind = 0, o_ek = 0,
+         o_gr = 0, o_pr = 0, o_br = 0, o_sd = 0;
+  if (host) {
+    if (need_aqs) {
+      o_aqsm = a.take((size_t)NB * NC * 16);
+      o_aqsn = a.take((size_t)NB * NC * 16);
+      o_vc = a.take((size_t)NC * 8);
+      o_inv = a.take((size_t)NG * 4);
+      o_ind = a.take((size_t)NC * 4);
+    }
+    if (do_ach) {
+      o_ek = a.take((size_t)NB * 8);
+      o_gr = a.take((size_t)nF * 8);
+      o_pr = a.take((size_t)nF * 8);
+      o_br = a.take((size_t)nF * 16);
+      o_sd = a.take((size_t)NB * nF * 16);
+    }
+  }
+  if ((rc = ensure(h, h->ff, a.used))) return rc;
+  if (host && !slices.empty() &&
+      (rc = ensure(h, h->ff_stage, slices.size() * per_slice * 16)))
+    return rc;
+  char *base = (char *)h->ff.ptr;
+  const double2 *aqsm = (const double2 *)p->aqsmtemp;
+  const double2 *aqsn = (const double2 *)p->aqsntemp;
+  const double *vcoul = p->vcoul;
+  const int *inv = p->inv_igp_index, *indinv = p->indinv;
+  gpp::ff::FreqArgs fa{};
+  fa.ekq = p->ekq;
+  fa.grid = p->dFreqGrid;
+  fa.pref = p->pref;
+  fa.brd = (const double2 *)p->dFreqBrd;
+  fa.schdt = (const double2 *)p->schDt_matrix;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const int kind = slices[job_slice[j]].first, f = slices[job_slice[j]].second;
+    if (host)
+      jobs[j].slice = (const double2 *)h->ff_stage.ptr + job_slice[j] * per_slice;
+    else
+      jobs[j].slice = (const double2 *)(kind == kR ? p->I_epsR_array
+                                                   : p->I_epsA_array) +
+                      (size_t)f * per_slice;
+  }
+  if (host) {
+    gpp::Stager *st = gpp::stager(h, &rc);
+    if (!st) return rc;
+    auto put = [&](size_t off, const void *src, size_t bytes) -> int {
+      if (bytes && src) HIP_OR_RET(h, st->copy(base + off, src, bytes));
+      return GPP_OK;
+    };
+    if (need_aqs) {
+      if ((rc = put(o_aqsm, p->aqsmtemp, (size_t)NB * NC * 16)) ||
+          (rc = put(o_aqsn, p->aqsntemp, (size_t)NB * NC * 16)) ||
+          (rc = put(o_vc, p->vcoul, (size_t)NC * 8)) ||
+          (rc = put(o_inv, p->inv_igp_index, (size_t)NG * 4)) ||
+          (rc = put(o_ind, p->indinv, (size_t)NC * 4)))
+        return rc;
+      aqsm = (const double2 *)(base + o_aqsm);
+      aqsn = (const double2 *)(base + o_aqsn);
+      vcoul = (const double *)(base + o_vc);
+      inv = (const int *)(base + o_inv);
+      indinv = (const int *)(base + o_ind);
+    }
+    if (do_ach) {
+      if ((rc = put(o_ek, p->ekq, (size_t)NB * 8)) ||
+          (rc = put(o_gr, p->dFreqGrid, (size_t)nF * 8)) ||
+          (rc = put(o_pr, p->pref, (size_t)nF * 8)) ||
+          (rc = put(o_br, p->dFreqBrd, (size_t)nF * 16)) ||
+          (rc = put(o_sd, p->schDt_matrix, (size_t)NB * nF * 16)))
+        return rc;
+      fa.ekq = (const double *)(base + o_ek);
+      fa.grid = (const double *)(base + o_gr);
+      fa.pref = (const double *)(base + o_pr);
+      fa.brd = (const double2 *)(base + o_br);
+      fa.schdt = (const double2 *)(base + o_sd);
+    }
+    for (size_t i = 0; i < slices.size(); ++i) {
+      const gpp_c128 *src =
+          (slices[i].first == kR ? p->I_epsR_array : p->I_epsA_array) +
+          (size_t)slices[i].second * per_slice;
+      HIP_OR_RET(h, st->copy((double2 *)h->ff_stage.ptr + i * per_slice, src,
+                             per_slice * 16));
+    }
+    HIP_OR_RET(h, st->fence(s));
+  }
+  if (njobs > 0)
+    HIP_OR_RET(h, hipMemcpyAsync(base + o_jobs, jobs.data(),
+                                 jobs.size() * sizeof(SliceJob),
+                                 hipMemcpyHostToDevice, s));
+  const size_t nout = (size_t)njobs * gpp::ff::kRecSlots +
+                      (do_ach ? (size_t)NE * gpp::ff::kFreqSlots : 0);
+  if ((rc = ensure_host_out(h, nout + 1))) return rc;
+
+  // ---- kernels
+  HIP_OR_RET(h, hipEventRecord(h->ev[0], s));
+  if (njobs > 0) {
+    double2 *cG = (double2 *)(base + o_cg);
+    HIP_OR_RET(h, gpp::ff::launch_cg(aqsm, vcoul, inv, indinv, NB, NC, NG, cG,
+                                     s));
+    HIP_OR_RET(h, gpp::ff::launch_slices((const SliceJob *)(base + o_jobs),
+                                         njobs, cG, aqsn, NC, NG, ngroups,
+                                         (double *)(base + o_rec), s));
+    HIP_OR_RET(h, gpp::ff::launch_slice_reduce((const double *)(base + o_rec),
+                                               njobs, per_job,
+                                               (double *)(base + o_T), s));
+  }
+  HIP_OR_RET(h, hipEventRecord(h->ev[1], s));
+  if (do_ach) {
+    fa.NB = NB;
+    fa.NV = NV;
+    fa.nFreq = nF;
+    fa.NE = NE;
+    fa.fmin = p->freqevalmin;
+    fa.fstep = p->freqevalstep;
+    fa.pref_zb = p->pref_zb;
+    HIP_OR_RET(h, gpp::ff::launch_freq(fa, sem, (double *)(base + o_part), s));
+    HIP_OR_RET(h, gpp::ff::launch_freq_reduce((const double *)(base + o_part),
+                                              NB, NE,
+                                              (double *)(base + o_fout), s));
+  }
+  HIP_OR_RET(h, hipEventRecord(h->ev[2], s));
+  double *ho = h->host_out;
+  if (njobs > 0)
+    HIP_OR_RET(h, hipMemcpyAsync(ho, base + o_T,
+                                 (size_t)njobs * gpp::ff::kRecSlots * 8,
+                                 hipMemcpyDeviceToHost, s));
+  double *hf = ho + (size_t)njobs * gpp::ff::kRecSlots;
+  if (do_ach)
+    HIP_OR_RET(h, hipMemcpyAsync(hf, base + o_fout,
+                                 (size_t)NE * gpp::ff::kFreqSlots * 8,
+                                 hipMemcpyDeviceToHost, s));
+  HIP_OR_RET(h, hipStreamSynchronize(s));
+
+  // ---- assembly
+  auto T = [&](int kind, int f, int b) -> Cplx {
+    const int slot = need[((size_t)kind * nF + f) * NB + b];
+    Cplx c;
+    c.re = ho[(size_t)slot * 2];
+    c.im = ho[(size_t)slot * 2 + 1];
+    return c;
+  };
+  auto interp = [&](int kind, const Fact &fc, int b) -> Cplx {
+    const Cplx x = T(kind, fc.f, b), y = T(kind, fc.f + 1, b);
+    Cplx c;
+    c.re = fc.f1 * x.re + fc.f2 * y.re;
+    c.im = fc.f1 * x.im + fc.f2 * y.im;
+    return c;
+  };
+  Cplx achs;
+  if ((parts & FF_PART_ACHS) && sums_possible)
+    for (int b = 0; b < NB; ++b) {
+      const Cplx t = T(kR, achs_slice, b);
+      achs.add(-0.5 * t.re, -0.5 * t.im);
+    }
+  r->achsDtemp.re = achs.re;
+  r->achsDtemp.im = achs.im;
+  std::vector<Cplx> asx(NE), cor(NE), a2(NE);
+  if ((parts & FF_PART_ASX) && sums_possible)
+    for (int b = 0; b < NV; ++b)
+      for (int iw = 0; iw < NE; ++iw) {
+        const Fact &fc = fasx[(size_t)b * NE + iw];
+        if (fc.f < 0) continue;
+        const Cplx v = interp(kasx[(size_t)b * NE + iw], fc, b);
+        asx[iw].add(p->occ * v.re, p->occ * v.im);
+      }
+  if ((parts & FF_PART_COR) && sums_possible)
+    for (int b = 0; b < NB; ++b)
+      for (int iw = 0; iw < NE; ++iw) {
+        const Fact &fc = fcor[(size_t)b * NE + iw];
+        if (fc.f < 0) continue;
+        const Cplx x = interp(kR, fc, b), y = interp(kA, fc, b);
+        Cplx v;
+        v.re = x.re - y.re;
+        v.im = x.im - y.im;
+        if (pcor[(size_t)b * NE + iw]) {
+          a2[iw].add(v.re, v.im);
+          if (b >= NV) cor[iw].add(v.re, v.im);
+        } else {
+          cor[iw].add(-0.5 * v.re, -0.5 * v.im);
+        }
+      }
+  for (int iw = 0; iw < NE; ++iw) {
+    const double *q = do_ach ? hf + (size_t)iw * gpp::ff::kFreqSlots : nullptr;
+    if (r->asxDtemp) r->asxDtemp[iw] = gpp_c128{asx[iw].re, asx[iw].im};
+    if (r->achDtemp) r->achDtemp[iw] = q ? gpp_c128{q[0], q[1]} : gpp_c128{0, 0};
+    if (r->achDtemp_corb)
+      r->achDtemp_corb[iw] = q ? gpp_c128{q[2], q[3]} : gpp_c128{0, 0};
+    if (r->ach2Dtemp)
+      r->ach2Dtemp[iw] = gpp_c128{a2[iw].re + (q ? q[4] : 0.0),
+                                  a2[iw].im + (q ? q[5] : 0.0)};
+    if (r->achDtemp_cor)
+      r->achDtemp_cor[iw] = gpp_c128{cor[iw].re + (q ? q[6] : 0.0),
+                                     cor[iw].im + (q ? q[7] : 0.0)};
+  }
+  float ms = 0.f;
+  HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+  r->slice_kernel_ms = ms;
+  HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
+  r->freq_kernel_ms = ms;
+  HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[2]));
+  r->total_ms = ms;
+  long long nslice_bands = 0;
+  for (const SliceJob &j : jobs) nslice_bands += j.nb;
+  r->slices = (long long)slices.size();
+  r->slice_bytes = (double)jobs.size() * per_slice * 16.0;
+  r->slice_terms = nslice_bands * (long long)per_slice;
+  r->freq_terms = do_ach ? (long long)NB * nF * NE : 0;
+  return GPP_OK;
+}
+
+// ---- drop-ins for FFKer_cori/ffKernel.cpp (FF_SEM_CPP, host memory)
+
+namespace {
+
+int run_cpp(const ff_problem &p, int parts, ff_result *r) {
+  int rc;
+  gpp_handle h = gpp::default_handle(&rc);
+  if (rc) return rc;
+  ff_options o;
+  ff_default_options(&o);
+  o.parts = parts;
+  return ff_sigma_ex(h, &p, &o, r);
+}
+
+void add_into(gpp_c128 *dst, const std::vector<gpp_c128> &v) {
+  if (!dst) return;
+  for (size_t i = 0; i < v.size(); ++i) {
+    dst[i].re += v[i].re;
+    dst[i].im += v[i].im;
+  }
+}
+
+}  // namespace
+
+int ff_achsDtemp_Kernel(int number_bands, int ngpown, int ncouls,
+                        const int *inv_igp_index, const int *indinv,
+                        const gpp_c128 *aqsntemp, const gpp_c128 *aqsmtemp,
+                        const gpp_c128 *I_epsR_array, const double *vcoul,
+                        gpp_c128 *achsDtemp) {
+  if (!achsDtemp) return GPP_EINVAL;
+  ff_problem p{};
+  p.number_bands = number_bands;
+  p.ncouls = ncouls;
+  p.ngpown = ngpown;
+  p.nFreq = 2;  // the kernel reads I_epsR slice 1 only
+  p.aqsmtemp = aqsmtemp;
+  p.aqsntemp = aqsntemp;
+  p.I_epsR_array = I_epsR_array;
+  p.vcoul = vcoul;
+  p.inv_igp_index = inv_igp_index;
+  p.indinv = indinv;
+  ff_result r{};
+  const int rc = run_cpp(p, FF_PART_ACHS, &r);
+  if (rc) return rc;
+  *achsDtemp = r.achsDtemp;  // :128 assigns
+  return GPP_OK;
+}
+
+int ff_asxDtemp_Kernel(int nvband, int nfreqeval, int ncouls, int ngpown,
+                       int numThreads, int nFreq, double freqevalmin,
+                       double freqevalstep, double occ, const double *ekq,
+                       const double *dFreqGrid, const int *inv_igp_index,
+                       const int *indinv, const gpp_c128 *aqsmtemp,
+                       const gpp_c128 *aqsntemp, const double *vcoul,
+                       const gpp_c128 *I_epsR_array, gpp_c128 *ssxDittt,
+                       gpp_c128 *asxDtemp) {
+  (void)numThreads;
+  (void)ssxDittt;
+  if (!asxDtemp || nfreqeval < 0) return GPP_EINVAL;
+  ff_problem p{};
+  p.number_bands = nvband;  // only bands n1 < nvband are read
+  p.nvband = nvband;
+  p.ncouls = ncouls;
+  p.ngpown = ngpown;
+  p.nFreq = nFreq;
+  p.nfreqeval = nfreqeval;
+  p.aqsmtemp = aqsmtemp;
+  p.aqsntemp = aqsntemp;
+  p.I_epsR_array = I_epsR_array;
+  p.vcoul = vcoul;
+  p.inv_igp_index = inv_igp_index;
+  p.indinv = indinv;
+  p.ekq = ekq;
+  p.dFreqGrid = dFreqGrid;
+  p.freqevalmin = freqevalmin;
+  p.freqevalstep = freqevalstep;
+  p.occ = occ;
+  std::vector<gpp_c128> out(nfreqeval);
+  ff_result r{};
+  r.asxDtemp = out.data();
+  const int rc = run_cpp(p, FF_PART_ASX, &r);
+  if (rc) return rc;
+  add_into(asxDtemp, out);  // :151 adds
+  return GPP_OK;
+}
+
+int ff_achDtemp_Kernel(int number_bands, int nvband, int nfreqeval,
+                       int ncouls, int ngpown, int nFreq, double freqevalmin,
+                       double freqevalstep, const double *ekq, double pref_zb,
+                       const double *pref, const double *dFreqGrid,
+                       const gpp_c128 *dFreqBrd, const gpp_c128 *schDt_matrix,
+                       gpp_c128 *schDi, gpp_c128 *schDi_cor, gpp_c128 *sch2Di,
+                       gpp_c128 *achDtemp) {
+  (void)schDi;
+  (void)schDi_cor;
+  (void)sch2Di;
+  if (!achDtemp || nfreqeval < 0) return GPP_EINVAL;
+  ff_problem p{};
+  p.number_bands = number_bands;
+  p.nvband = nvband;
+  p.ncouls = ncouls;
+  p.ngpown = ngpown;
+  p.nFreq = nFreq;
+  p.nfreqeval = nfreqeval;
+  p.ekq = ekq;
+  p.dFreqGrid = dFreqGrid;
+  p.pref = pref;
+  p.dFreqBrd = dFreqBrd;
+  p.schDt_matrix = schDt_matrix;
+  p.freqevalmin = freqevalmin;
+  p.freqevalstep = freqevalstep;
+  p.pref_zb = pref_zb;
+  std::vector<gpp_c128> out(nfreqeval);
+  ff_result r{};
+  r.achDtemp = out.data();
+  const int rc = run_cpp(p, FF_PART_ACH, &r);
+  if (rc) return rc;
+  add_into(achDtemp, out);  // :206 adds
+  return GPP_OK;
+}
+
+int ff_achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval,
+                           int ncouls, int ngpown, int nFreq,
+                           double freqevalmin, double freqevalstep,
+                           const double *ekq, const double *dFreqGrid,
+                           const int *inv_igp_index, const int *indinv,
+                           const gpp_c128 *aqsmtemp, const gpp_c128 *aqsntemp,
+                           const double *vcoul, const gpp_c128 *I_epsR_array,
+                           const gpp_c128 *I_epsA_array, gpp_c128 *schDi_cor,
+                           gpp_c128 *schDi_corb, gpp_c128 *sch2Di,
+                           gpp_c128 *ach2Dtemp, gpp_c128 *achDtemp_cor,
+                           gpp_c128 *achDtemp_corb) {
+  (void)schDi_cor;
+  (void)schDi_corb;
+  (void)sch2Di;
+  if (nfreqeval < 0) return GPP_EINVAL;
+  ff_problem p{};
+  p.number_bands = number_bands;
+  p.nvband = nvband;
+  p.ncouls = ncouls;
+  p.ngpown = ngpown;
+  p.nFreq = nFreq;
+  p.nfreqeval = nfreqeval;
+  p.aqsmtemp = aqsmtemp;
+  p.aqsntemp = aqsntemp;
+  p.I_epsR_array = I_epsR_array;
+  p.I_epsA_array = I_epsA_array;
+  p.vcoul = vcoul;
+  p.inv_igp_index = inv_igp_index;
+  p.indinv = indinv;
+  p.ekq = ekq;
+  p.dFreqGrid = dFreqGrid;
+  p.freqevalmin = freqevalmin;
+  p.freqevalstep = freqevalstep;
+  std::vector<gpp_c128> cor(nfreqeval), a2(nfreqeval);
+  ff_result r{};
+  r.achDtemp_cor = cor.data();
+  r.ach2Dtemp = a2.data();
+  const int rc = run_cpp(p, FF_PART_COR, &r);
+  if (rc) return rc;
+  add_into(achDtemp_cor, cor);  // :241-243 add; achDtemp_corb gets zeros
+  add_into(ach2Dtemp, a2);
+  (void)achDtemp_corb;
+  return GPP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// ff_internal.h — shared between the FF kernels (ff_device.hip) and the FF
+// host side (ff_api.cpp).  Not public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace gpp {
+namespace ff {
+
+constexpr int kMaxJobBands = 8;   // bands sharing one pass over a slice
+constexpr int kSliceThreads = 256;
+constexpr int kRecSlots = 2 * kMaxJobBands;  // complex per band per record
+constexpr int kFreqSlots = 8;     // ach, corb, sch2 (grid part), cor (grid
+                                  // part): complex each
+
+// One pass over one frequency slice of I_epsR or I_epsA for up to 8 bands:
+//   T[b] = sum_{G'} cG[band_b][G'] sum_G aqsn[band_b][G] * slice[G'][G]
+// with cG[n1][G'] = vcoul[igp] * conj(aqsm[n1][igp]) (0 for skipped rows).
+struct SliceJob {
+  const double2 *slice;  // [NG][NC]
+  int nb;
+  int band[kMaxJobBands];
+  int pad;
+};
+
+// Scalars of the frequency-grid kernel.
+struct FreqArgs {
+  const double *ekq;        // [NB]
+  const double *grid;       // [nFreq]
+  const double *pref;       // [nFreq]
+  const double2 *brd;       // [nFreq]
+  const double2 *schdt;     // [NB][nFreq]
+  int NB, NV, nFreq, NE;
+  double fmin, fstep, pref_zb;
+};
+
+hipError_t launch_cg(const double2 *aqsm, const double *vcoul,
+                     const int *inv_igp_index, const int *indinv, int NB,
+                     int NC, int NG, double2 *cG, hipStream_t s);
+// Grid: jobs x row groups x G chunks; rec[(job*ngroups + g)*nchunk + c][16].
+hipError_t launch_slices(const SliceJob *jobs, int njobs, const double2 *cG,
+                         const double2 *aqsn, int NC, int NG, int ngroups,
+                         double *rec, hipStream_t s);
+// T[job][16] = fixed-order sum of the job's ngroups*nchunk records.
+hipError_t launch_slice_reduce(const double *rec, int njobs, int per_job,
+                               double *T, hipStream_t s);
+int slice_chunks(int NC);
+// part[(n1*NE + iw)*8]; sem 0 = ffKernel.cpp, 1 = ffKernel.f90.
+hipError_t launch_freq(const FreqArgs &a, int sem, double *part,
+                       hipStream_t s);
+// out[iw*8] = sum over n1 (in order) of part.
+hipError_t launch_freq_reduce(const double *part, int NB, int NE, double *out,
+                              hipStream_t s);
+
+// Synthetic I_eps slices on the device (config 1: constants; config 2: the
+// counter-based values of ff_fill_inputs).
+hipError_t launch_fill_eps(int config, uint64_t seed, int nFreq, int NG,
+                           int NC, double2 *epsR, double2 *epsA,
+                           hipStream_t s);
+hipError_t launch_fill_aqs(int config, uint64_t seed, int NB, int NC,
+                           double2 *aqsm, double2 *aqsn, hipStream_t s);
+
+}  // namespace ff
+}  // namespace gpp

@@ -27,22 +27,21 @@
 #include <vector>
 
 #include "../../include/gpp_sigma.h"
-#include "gpp_internal.h"
-#include "gpp_stage.h"
+#include "gpp_handle.h"
 
 using gpp::BatchItem;
 using gpp::DevProblem;
 using gpp::TileCfg;
 
+using gpp::Arena;
+using gpp::DevBuf;
+using gpp::ensure;
+using gpp::ensure_host_out;
+using gpp::set_err;
+using gpp::stager;
+using gpp::TuneKey;
+
 namespace {
-
-struct DevBuf {
-  void *ptr = nullptr;
-  size_t cap = 0;
-};
-
-// Tuned tile per shape class: (precision, masked, log2 NC, log2 NG).
-using TuneKey = std::tuple<int, int, int, int>;
 
 // Mixed precision: |wx - wtilde|^2 below which a term is redone in FP64.
 // The FP32 path forms wx - wtilde in FP64 before rounding, so its relative
@@ -52,27 +51,7 @@ constexpr double kDefaultResonance = 1e-6;
 
 }  // namespace
 
-struct gpp_handle_s {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[6] = {};
-  DevBuf stage;                 // host-input staging arena
-  DevBuf work;                  // per-call workspace arena
-  DevBuf part;                  // block records
-  DevBuf n1log;                 // per-band records (acht_n1_loc)
-  DevBuf meta;                  // batch: problem table, items, record ranges
-  double *host_out = nullptr;   // pinned result slots
-  size_t host_out_cap = 0;      // doubles
-  std::map<TuneKey, TileCfg> tuned;
-  std::string tune_file;
-  // Cached graph of the last batch (replayed when the call is identical).
-  hipGraphExec_t graph_exec = nullptr;
-  std::vector<unsigned char> graph_sig;
-  std::string err;
-  gpp::Stager *stager = nullptr;  // host-input copies (created on first use)
-};
-
-namespace {
+namespace gpp {
 
 int set_err(gpp_handle h, int code, const char *fmt, ...) {
   if (h) {
@@ -85,16 +64,6 @@ int set_err(gpp_handle h, int code, const char *fmt, ...) {
   }
   return code;
 }
-
-#define HIP_OR_RET(h, expr)                                                  \
-  do {                                                                       \
-    hipError_t e_ = (expr);                                                  \
-    if (e_ != hipSuccess)                                                    \
-      return set_err((h), e_ == hipErrorOutOfMemory ? GPP_ENOMEM             \
-                                                    : GPP_EDEVICE,           \
-                     "%s: %s (%s:%d)", #expr, hipGetErrorString(e_),         \
-                     __FILE__, __LINE__);                                    \
-  } while (0)
 
 int ensure(gpp_handle h, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 256;
@@ -125,15 +94,34 @@ int ensure_host_out(gpp_handle h, size_t doubles) {
   return GPP_OK;
 }
 
-// Bump allocator over one DevBuf, 256-byte aligned slices.
-struct Arena {
-  size_t used = 0;
-  size_t take(size_t bytes) {
-    const size_t at = used;
-    used += (bytes + 255) / 256 * 256;
-    return at;
+Stager *stager(gpp_handle h, int *rc) {
+  *rc = GPP_OK;
+  if (!h->stager) h->stager = new (std::nothrow) Stager;
+  if (!h->stager) {
+    *rc = set_err(h, GPP_ENOMEM, "stager allocation failed");
+    return nullptr;
   }
-};
+  hipError_t e = h->stager->init();
+  if (e != hipSuccess) {
+    *rc = set_err(h, e == hipErrorOutOfMemory ? GPP_ENOMEM : GPP_EDEVICE,
+                  "pinned staging ring: %s", hipGetErrorString(e));
+    return nullptr;
+  }
+  return h->stager;
+}
+
+gpp_handle default_handle(int *rc) {
+  static std::mutex mu;
+  static gpp_handle h = nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  *rc = GPP_OK;
+  if (!h) *rc = gpp_create(&h, -1);
+  return h;
+}
+
+}  // namespace gpp
+
+namespace {
 
 int validate(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
   if (!p || !o) return set_err(h, GPP_EINVAL, "null problem or options");
@@ -262,22 +250,6 @@ struct Prepared {
   const double *wx_in = nullptr;
   int wx_per_band = 0;
 };
-
-gpp::Stager *stager(gpp_handle h, int *rc) {
-  *rc = GPP_OK;
-  if (!h->stager) h->stager = new (std::nothrow) gpp::Stager;
-  if (!h->stager) {
-    *rc = set_err(h, GPP_ENOMEM, "stager allocation failed");
-    return nullptr;
-  }
-  hipError_t e = h->stager->init();
-  if (e != hipSuccess) {
-    *rc = set_err(h, e == hipErrorOutOfMemory ? GPP_ENOMEM : GPP_EDEVICE,
-                  "pinned staging ring: %s", hipGetErrorString(e));
-    return nullptr;
-  }
-  return h->stager;
-}
 
 // Plans the device layout of `n` problems: staging (host inputs) in
 // h->stage, igp/wx/aqsn32 tables in h->work.  Host inputs are copied
@@ -456,7 +428,8 @@ int gpp_destroy(gpp_handle h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-  DevBuf *bufs[] = {&h->stage, &h->work, &h->part, &h->n1log, &h->meta};
+  DevBuf *bufs[] = {&h->stage, &h->work, &h->part, &h->n1log,
+                    &h->meta,  &h->ff,   &h->ff_stage};
   for (DevBuf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   for (auto &ev : h->ev)
@@ -879,16 +852,6 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   return GPP_OK;
 }
 
-// Library-owned handle for the reference-shaped entry points.
-static gpp_handle default_handle(int *rc) {
-  static std::mutex mu;
-  static gpp_handle h = nullptr;
-  std::lock_guard<std::mutex> lock(mu);
-  *rc = GPP_OK;
-  if (!h) *rc = gpp_create(&h, -1);
-  return h;
-}
-
 int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
               const gpp_c128 *aqsmtemp, const gpp_c128 *aqsntemp,
               const gpp_c128 *I_eps_array, const gpp_c128 *wtilde_array,
@@ -897,7 +860,7 @@ int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
               gpp_c128 *asxtemp) {
   if (!achtemp || !asxtemp) return GPP_EINVAL;
   int rc;
-  gpp_handle h = default_handle(&rc);
+  gpp_handle h = gpp::default_handle(&rc);
   if (rc) return rc;
   gpp_problem p = {number_bands, nvband,   ncouls,        ngpown,
                    aqsmtemp,     aqsntemp, I_eps_array,   wtilde_array,
@@ -928,7 +891,7 @@ int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
       !achtemp_im1 || !achtemp_im2)
     return GPP_EINVAL;
   int rc;
-  gpp_handle h = default_handle(&rc);
+  gpp_handle h = gpp::default_handle(&rc);
   if (rc) return rc;
   gpp_problem p = {number_bands, 0,        ncouls,        ngpown,
                    aqsmtemp,     aqsntemp, I_eps_array,   wtilde_array,
@@ -960,7 +923,7 @@ int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
                           int nend, double occ, int number_bands) {
   if (!asxtemp || nstart < 0 || nend > 3 || nstart > nend) return GPP_EINVAL;
   int rc;
-  gpp_handle h = default_handle(&rc);
+  gpp_handle h = gpp::default_handle(&rc);
   if (rc) return rc;
   gpp_problem p = {number_bands, nvband,   ncouls,        ngpown,
                    aqsmtemp,     aqsntemp, I_eps_array,   wtilde_array,

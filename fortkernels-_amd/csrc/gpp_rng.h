@@ -1,5 +1,6 @@
-// gpp_rng.h — counter-based random numbers shared by the host generator
-// (gpp_inputs.cpp) and the device generator (gpp_gen.hip).  Every draw is
+// gpp_rng.h — counter-based random numbers shared by the host generators
+// (gpp_inputs.cpp, ff_gen.hip) and the device generators (gpp_gen.hip,
+// ff_gen.hip).  Every draw is
 // a pure function of (seed, stream, index): SplitMix64 finaliser over a
 // keyed counter; integer arithmetic only, so host and device agree bit for
 // bit on every uniform draw.
@@ -24,6 +25,15 @@ enum Stream : uint64_t {
   kEnergy = 5,
   kResPick = 6,
   kResWhere = 7,
+  // full-frequency (FF) inputs
+  kFFAqsm = 11,
+  kFFAqsn = 12,
+  kFFEpsR = 13,
+  kFFEpsA = 14,
+  kFFEkq = 15,
+  kFFGrid = 16,
+  kFFBrd = 17,
+  kFFSchDt = 18,
 };
 
 GPP_HD inline uint64_t mix64(uint64_t z) {

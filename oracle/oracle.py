@@ -128,3 +128,83 @@ def resolve_igp(inp: dict) -> np.ndarray:
     return np.array([lib.gpp_oracle_resolve_igp(i, nc, inv.ctypes.data,
                                                 ind.ctypes.data)
                      for i in range(inv.shape[0])], np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Full-frequency (FF) kernels: ff_oracle.c.
+
+FF_SEM = {"cpp": 0, "f90": 1}
+FF_OUTPUTS = ("asxDtemp", "achDtemp", "achDtemp_cor", "achDtemp_corb", "ach2Dtemp")
+
+
+class _FFProblem(C.Structure):
+    _fields_ = [
+        ("number_bands", C.c_int), ("nvband", C.c_int), ("ncouls", C.c_int),
+        ("ngpown", C.c_int), ("nFreq", C.c_int), ("nfreqeval", C.c_int),
+        ("aqsmtemp", C.c_void_p), ("aqsntemp", C.c_void_p),
+        ("I_epsR_array", C.c_void_p), ("I_epsA_array", C.c_void_p),
+        ("vcoul", C.c_void_p), ("inv_igp_index", C.c_void_p),
+        ("indinv", C.c_void_p), ("ekq", C.c_void_p), ("dFreqGrid", C.c_void_p),
+        ("pref", C.c_void_p), ("dFreqBrd", C.c_void_p),
+        ("schDt_matrix", C.c_void_p),
+        ("freqevalmin", C.c_double), ("freqevalstep", C.c_double),
+        ("pref_zb", C.c_double), ("occ", C.c_double),
+    ]
+
+
+_FF_ARRAYS = {
+    "aqsmtemp": np.complex128, "aqsntemp": np.complex128,
+    "I_epsR_array": np.complex128, "I_epsA_array": np.complex128,
+    "vcoul": np.float64, "inv_igp_index": np.int32, "indinv": np.int32,
+    "ekq": np.float64, "dFreqGrid": np.float64, "pref": np.float64,
+    "dFreqBrd": np.complex128, "schDt_matrix": np.complex128,
+}
+
+
+def _ff_problem(inp: dict):
+    keep = {k: _c(inp[k], t) for k, t in _FF_ARRAYS.items()}
+    p = _FFProblem(int(inp["number_bands"]), int(inp["nvband"]),
+                   int(inp["ncouls"]), int(inp["ngpown"]), int(inp["nFreq"]),
+                   int(inp["nfreqeval"]),
+                   *[keep[k].ctypes.data for k in _FF_ARRAYS],
+                   float(inp["freqevalmin"]), float(inp["freqevalstep"]),
+                   float(inp["pref_zb"]), float(inp["occ"]))
+    return p, keep
+
+
+def _ff_load():
+    lib = load()
+    if not hasattr(lib, "_ff_ready"):
+        lib.ff_oracle.argtypes = [C.c_int, C.POINTER(_FFProblem)] + [C.c_void_p] * 6
+        lib.ff_oracle.restype = C.c_int
+        lib.ff_oracle_fact.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_int, C.c_void_p, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+        lib.ff_oracle_fact.restype = C.c_int
+        lib._ff_ready = True
+    return lib
+
+
+def ff_sigma(inp: dict, semantics: str = "cpp") -> dict:
+    """All FF outputs: achsDtemp (complex) and the [nfreqeval] arrays."""
+    lib = _ff_load()
+    p, keep = _ff_problem(inp)
+    ne = int(inp["nfreqeval"])
+    achs = np.zeros(1, np.complex128)
+    outs = {k: np.zeros(ne, np.complex128) for k in FF_OUTPUTS}
+    rc = lib.ff_oracle(FF_SEM[semantics], C.byref(p), achs.ctypes.data,
+                       *[outs[k].ctypes.data for k in FF_OUTPUTS])
+    if rc:
+        raise ValueError(f"ff_oracle rc={rc}")
+    outs["achsDtemp"] = complex(achs[0])
+    return outs
+
+
+def ff_fact(semantics: str, loop: int, flag_occ: bool, wx: float, grid) -> tuple:
+    """(ifreq, fact1, fact2) of the variant's frequency search."""
+    lib = _ff_load()
+    g = _c(grid, np.float64)
+    f1, f2 = C.c_double(), C.c_double()
+    f = lib.ff_oracle_fact(FF_SEM[semantics], loop, int(flag_occ), float(wx),
+                           len(g), g.ctypes.data, C.byref(f1), C.byref(f2))
+    return f, f1.value, f2.value

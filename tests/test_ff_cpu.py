@@ -1,0 +1,195 @@
+"""CPU tests of the full-frequency (FF) path: the oracle against the
+reference's own FF code (the unmodified mini-app binary and its kernel
+functions on random inputs), closed forms of the mini-app's fills, the
+frequency-grid search, invariants, the input generator and the CLI."""
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, gppk, gpp_oracle as orc
+
+
+def c(a):
+    a = np.asarray(a, np.float64)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def rel(a, b):
+    """max |a - b| / max |b| over the whole array (0 if both vanish)."""
+    a = np.atleast_1d(np.asarray(a, np.complex128))
+    b = np.atleast_1d(np.asarray(b, np.complex128))
+    s = np.max(np.abs(b))
+    d = np.max(np.abs(a - b))
+    return d if s == 0 else d / s
+
+
+def kat_problem(r):
+    return dict(number_bands=r["nb"], nvband=r["nv"], ncouls=r["nc"],
+                ngpown=r["ng"], nFreq=r["nf"], nfreqeval=r["ne"],
+                aqsmtemp=c(r["aqsm"]), aqsntemp=c(r["aqsn"]),
+                I_epsR_array=c(r["er"]), I_epsA_array=c(r["ea"]),
+                vcoul=np.array(r["vcoul"]),
+                inv_igp_index=np.array(r["inv"], np.int32),
+                indinv=np.array(r["indinv"], np.int32), ekq=np.array(r["ekq"]),
+                dFreqGrid=np.array(r["grid"]), pref=np.array(r["pref"]),
+                dFreqBrd=c(r["brd"]), schDt_matrix=c(r["schdt"]),
+                freqevalmin=r["fmin"], freqevalstep=r["fstep"],
+                pref_zb=r["pref_zb"], occ=r["occ"])
+
+
+KATS = json.loads((GOLDEN / "ref_ff_kat.json").read_text())
+CONST = json.loads((GOLDEN / "ref_ff_constfill.json").read_text())
+
+
+@pytest.mark.parametrize("r", KATS, ids=lambda r: f"seed{r['seed']}")
+def test_oracle_matches_reference_kernels(r):
+    """ffKernel.cpp's own achsDtemp_Kernel/asxDtemp_Kernel/achDtemp_Kernel/
+    achDtemp_cor_Kernel on random inputs (non-uniform grid, wx below zero,
+    inside and beyond the grid, strided and permuted index maps)."""
+    got = orc.ff_sigma(kat_problem(r), "cpp")
+    assert abs(got["achsDtemp"] - complex(*r["achsDtemp"])) <= 1e-13 * abs(complex(*r["achsDtemp"]))
+    for k in ("asxDtemp", "achDtemp", "achDtemp_cor"):
+        assert rel(got[k], c(r[k])) <= 1e-13, k
+
+
+def test_kat_inputs_exercise_both_cor_branches():
+    """At least one fixture has a non-zero achDtemp_cor, and the band energies
+    put wx on both sides of zero."""
+    assert any(np.any(c(r["achDtemp_cor"]) != 0) for r in KATS)
+    for r in KATS:
+        wx = r["fmin"] - np.array(r["ekq"]) + r["fstep"]
+        assert (wx > 0).any() and (wx <= 0).any()
+
+
+def closed_form(sem, nb, nv, nc, ng, nf, ne):
+    """The mini-app's fills (aqsm = aqsn = 0.5+0.5i, I_epsR = 0.5+0.5i,
+    I_epsA = 0.5-0.5i, vcoul = 1, identity maps, ekq = -10+n1, schDt = 0):
+    every interpolation has weights summing to 1 (asx) or -1/2 (cor, wx > 0),
+    aqsn*conj(aqsm) = 1/2, I_epsR - I_epsA = i."""
+    K = ng * nc
+    achs = -0.125 * nb * K * (1 + 1j)
+    asx = np.zeros(ne, complex)
+    cor = np.zeros(ne, complex)
+    a2 = np.zeros(ne, complex)
+    for n1 in range(nb):
+        occ = n1 < nv
+        for iw in range(ne):
+            wx = 0.0 - (-10.0 + n1) + (0.5 if sem == "cpp" else iw * 0.5)
+            if occ:
+                if sem == "cpp" or wx >= 0:
+                    asx[iw] += 0.25 * K * (1 + 1j)
+                else:
+                    asx[iw] += 0.25 * K * (1 - 1j)
+            pos = wx > 0 if sem == "cpp" else wx >= 0
+            s = -0.25j * K
+            if pos:
+                if sem == "f90" or not occ:
+                    a2[iw] += s
+                if not occ:
+                    cor[iw] += s
+            elif occ:
+                cor[iw] += s
+    return achs, asx, cor, a2
+
+
+SHAPES = [(4, 2, 64, 8, 6, 5), (14, 13, 40, 5, 9, 3), (3, 0, 30, 3, 4, 2),
+          (12, 3, 50, 6, 4, 4), (16, 14, 24, 4, 12, 30)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("sem", ["cpp", "f90"])
+def test_oracle_closed_form(shape, sem):
+    nb, nv, nc, ng, nf, ne = shape
+    inp = gppk.ff_fill_inputs(1, nb, nv, nc, ng, nf, ne, semantics=sem)
+    got = orc.ff_sigma(inp, sem)
+    achs, asx, cor, a2 = closed_form(sem, *shape)
+    assert abs(got["achsDtemp"] - achs) <= 1e-12 * abs(achs)
+    assert rel(got["asxDtemp"], asx) <= 1e-12
+    assert rel(got["achDtemp_cor"], cor) <= 1e-12
+    assert rel(got["ach2Dtemp"], a2) <= 1e-12
+    assert np.all(got["achDtemp"] == 0) and np.all(got["achDtemp_corb"] == 0)
+
+
+@pytest.mark.parametrize("r", CONST, ids=lambda r: "{number_bands}-{nvband}-{ncouls}-{ngpown}-{nFreq}-{nfreqeval}".format(**r))
+def test_oracle_matches_reference_binary_constfill(r):
+    """The lines the unmodified ffKernel.cpp prints (its main hands asxDtemp
+    to achDtemp_Kernel, so the printed asxDtemp includes achDtemp)."""
+    shape = [r[k] for k in ("number_bands", "nvband", "ncouls", "ngpown", "nFreq", "nfreqeval")]
+    inp = gppk.ff_fill_inputs(1, *shape, semantics="cpp")
+    got = orc.ff_sigma(inp, "cpp")
+    assert abs(got["achsDtemp"] - complex(*r["achsDtemp"])) <= 1e-6
+    assert abs(got["asxDtemp"][0] + got["achDtemp"][0] - complex(*r["asxDtemp"])) <= 1e-6
+    assert abs(got["achDtemp_cor"][0] - complex(*r["achDtemp_cor"])) <= 1e-6
+    achs, asx, cor, _ = closed_form("cpp", *shape)
+    assert abs(achs - complex(*r["achsDtemp"])) <= 1e-6
+    assert abs(asx[0] - complex(*r["asxDtemp"])) <= 1e-6
+    assert abs(cor[0] - complex(*r["achDtemp_cor"])) <= 1e-6
+
+
+def test_frequency_search_quirks():
+    g = 2.0 * np.arange(6)  # 0 2 4 6 8 10
+    # C++ loop 1: strict search; a hit in the first interval, a grid point
+    # and a point beyond the grid all fall back to the last interval.
+    assert orc.ff_fact("cpp", 1, False, 5.0, g) == (2, 0.5, 0.5)
+    assert orc.ff_fact("cpp", 1, False, 1.0, g)[0] == 4
+    assert orc.ff_fact("cpp", 1, False, 4.0, g)[0] == 4
+    f, f1, f2 = orc.ff_fact("cpp", 1, False, 10.5, g)
+    assert f == 4 and f1 + f2 == pytest.approx(1.0) and f1 < 0
+    f, f1, f2 = orc.ff_fact("cpp", 1, False, -5.0, g)  # uses -wx
+    assert (f, f1, f2) == (2, 0.5, 0.5)
+    # C++ loop 2, wx > 0: no fallback, weights sum to -1/2
+    f, f1, f2 = orc.ff_fact("cpp", 2, False, 1.0, g)
+    assert f == 0 and f1 + f2 == pytest.approx(-0.5)
+    assert orc.ff_fact("cpp", 2, True, 3.0, g)[0] == -1   # occupied, wx > 0
+    assert orc.ff_fact("cpp", 2, False, -3.0, g)[0] == -1  # unoccupied, wx <= 0
+    # F90: half-open search, last interval when nothing matches
+    assert orc.ff_fact("f90", 1, True, 4.0, g)[0] == 2
+    assert orc.ff_fact("f90", 1, True, 0.0, g)[0] == 0
+    assert orc.ff_fact("f90", 1, True, 11.0, g)[0] == 4
+    assert orc.ff_fact("f90", 1, False, 4.0, g)[0] == -1  # asx: occupied only
+    assert orc.ff_fact("f90", 2, False, -1.0, g)[0] == -1
+
+
+def test_linearity_in_schdt_and_aqsn():
+    inp = gppk.ff_fill_inputs(2, 5, 2, 40, 5, 6, 7, semantics="f90", seed=9)
+    base = orc.ff_sigma(inp, "f90")
+    two = orc.ff_sigma(dict(inp, schDt_matrix=2 * inp["schDt_matrix"]), "f90")
+    for k in ("achDtemp", "achDtemp_corb"):
+        assert rel(two[k], 2 * base[k]) <= 1e-13, k
+    three = orc.ff_sigma(dict(inp, aqsntemp=3 * inp["aqsntemp"]), "f90")
+    for k in ("asxDtemp",):
+        assert rel(three[k], 3 * base[k]) <= 1e-13, k
+    assert abs(three["achsDtemp"] - 3 * base["achsDtemp"]) <= 1e-13 * abs(base["achsDtemp"])
+
+
+def test_band_additivity_of_grid_terms():
+    """achDtemp is a sum over bands: splitting the bands splits it."""
+    inp = gppk.ff_fill_inputs(2, 6, 3, 30, 3, 5, 4, semantics="cpp", seed=4)
+    full = orc.ff_sigma(inp, "cpp")["achDtemp"]
+    parts = []
+    for a, b in ((0, 2), (2, 6)):
+        d = dict(inp, number_bands=b - a, nvband=max(0, min(3, b) - a),
+                 aqsmtemp=inp["aqsmtemp"][a:b], aqsntemp=inp["aqsntemp"][a:b],
+                 ekq=inp["ekq"][a:b], schDt_matrix=inp["schDt_matrix"][a:b])
+        parts.append(orc.ff_sigma(d, "cpp")["achDtemp"])
+    assert rel(parts[0] + parts[1], full) <= 1e-13
+
+
+def test_generator_is_deterministic_and_shaped():
+    a = gppk.ff_fill_inputs(2, 4, 2, 48, 6, 5, 3, seed=1)
+    b = gppk.ff_fill_inputs(2, 4, 2, 48, 6, 5, 3, seed=1)
+    for k in gppk.FF_ARRAYS:
+        assert np.array_equal(a[k], b[k]), k
+    assert a["I_epsR_array"].shape == (5, 6, 48)
+    assert np.all(np.diff(a["dFreqGrid"]) > 0) and np.all(np.diff(a["ekq"]) >= 0)
+    assert list(a["inv_igp_index"]) == [0, 8, 16, 24, 32, 40]
+    c1 = gppk.ff_fill_inputs(1, 4, 2, 48, 6, 5, 3, semantics="f90")
+    assert np.all(c1["dFreqBrd"] == 0.1j) and np.all(c1["I_epsA_array"] == 0.5 - 0.5j)
+
+
+def test_cli_usage():
+    out = subprocess.run([str(gppk.FF_CLI_PATH)], capture_output=True, text=True)
+    assert out.returncode == 0
+    assert "number_bands nvband ncouls ngpown nFreq nfreqeval" in out.stdout

@@ -52,9 +52,11 @@ def small_random(nb=6, nv=2, nc=48, npg=4, seed=3, config=2):
 
 def test_library_exports_every_declared_symbol():
     lib = gppk.load_library()
-    header = (ROOT / "include" / "gpp_sigma.h").read_text()
-    declared = set(re.findall(r"^\s*(?:int|void|const char \*)\s*(gpp_\w+)\(",
-                              header, re.M))
+    declared = set()
+    for name in ("gpp_sigma.h", "ff_sigma.h"):
+        header = (ROOT / "include" / name).read_text()
+        declared |= set(re.findall(
+            r"^\s*(?:int|void|const char \*)\s*((?:gpp|ff)_\w+)\(", header, re.M))
     assert declared == set(gppk.EXPORTS), declared ^ set(gppk.EXPORTS)
     for name in declared:
         assert hasattr(lib, name), name
