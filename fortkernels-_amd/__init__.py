@@ -135,6 +135,7 @@ class FFResult(C.Structure):
         ("freq_kernel_ms", C.c_double), ("total_ms", C.c_double),
         ("slice_bytes", C.c_double), ("slices", C.c_longlong),
         ("slice_terms", C.c_longlong), ("freq_terms", C.c_longlong),
+        ("host_plan_ms", C.c_double), ("host_assemble_ms", C.c_double),
     ]
 
 
@@ -348,7 +349,8 @@ class Handle:
             raise GppError(rc, self.last_error())
         outs["achsDtemp"] = _cplx(res.achsDtemp)
         for k in ("slice_kernel_ms", "freq_kernel_ms", "total_ms", "slice_bytes",
-                  "slices", "slice_terms", "freq_terms"):
+                  "slices", "slice_terms", "freq_terms", "host_plan_ms",
+                  "host_assemble_ms"):
             outs[k] = getattr(res, k)
         return outs
 

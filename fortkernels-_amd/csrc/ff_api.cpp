@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -73,6 +74,20 @@ int search_half_open(double x, int nF, const double *g, bool sorted) {
   return f < 0 ? nF - 2 : f;
 }
 
+// The same search for a strictly increasing grid when consecutive queries
+// move little (the evaluation frequencies of one band step by
+// freqevalstep): walk from the last interval instead of bisecting, which
+// keeps the host planning of 10^5-10^6 (band, frequency) pairs cheap.
+struct Walker {
+  int k = 0;
+  int find(double x, int nF, const double *g) {
+    if (!(x >= g[0] && x < g[nF - 1])) return nF - 2;
+    while (x < g[k]) --k;       // stops at k = 0 since x >= g[0]
+    while (x >= g[k + 1]) ++k;  // stops at k = nF-2 since x < g[nF-1]
+    return k;
+  }
+};
+
 Fact weights(int f, double x, double scale, const double *g) {
   Fact r;
   r.f = f;
@@ -82,10 +97,14 @@ Fact weights(int f, double x, double scale, const double *g) {
   return r;
 }
 
-// asx (loop 1) and cor (loop 2) weights of the two variants.
+// asx (loop 1) and cor (loop 2) weights of the two variants.  `w` (sorted
+// grids only) carries the F90 search position between calls.
 Fact fact(int sem, int loop, bool occ, double wx, int nF, const double *g,
-          bool sorted) {
+          bool sorted, Walker *w) {
   Fact r;
+  auto half_open = [&](double x) {
+    return (sorted && w) ? w->find(x, nF, g) : search_half_open(x, nF, g, sorted);
+  };
   if (sem == FF_SEM_CPP) {
     if (loop == 1) {  // called with flag_occ = 0 (ffKernel.cpp:146)
       const bool pos = wx > 0.0;
@@ -107,12 +126,12 @@ Fact fact(int sem, int loop, bool occ, double wx, int nF, const double *g,
   if (loop == 1) {  // occupied bands only
     if (!occ) return r;
     const double x = wx >= 0.0 ? wx : -wx;
-    return weights(search_half_open(x, nF, g, sorted), x, 1.0, g);
+    return weights(half_open(x), x, 1.0, g);
   }
   if (wx >= 0.0)
-    return weights(search_half_open(wx, nF, g, sorted), wx, -0.5, g);
+    return weights(half_open(wx), wx, -0.5, g);
   if (!occ) return r;
-  return weights(search_half_open(-wx, nF, g, sorted), -wx, 1.0, g);
+  return weights(half_open(-wx), -wx, 1.0, g);
 }
 
 struct Cplx {
@@ -190,6 +209,8 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   const int NC = p->ncouls, NG = p->ngpown, nF = p->nFreq, NE = p->nfreqeval;
   const size_t per_slice = (size_t)NG * NC;
 
+  using clk = std::chrono::steady_clock;
+  const auto t_plan = clk::now();
   // Host copies of the two small arrays the plan needs.
   std::vector<double> ekq(NB), grid(nF);
   const bool need_grid = parts & (FF_PART_ASX | FF_PART_ACH | FF_PART_COR);
@@ -216,14 +237,15 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   std::vector<Fact> fasx((size_t)NV * NE);
   std::vector<signed char> kasx((size_t)NV * NE, kR);
   if ((parts & FF_PART_ASX) && sums_possible)
-    for (int b = 0; b < NV; ++b)
+    for (int b = 0; b < NV; ++b) {
+      Walker walk;
       for (int iw = 0; iw < NE; ++iw) {
         const double wx = sem == FF_SEM_CPP
                               ? p->freqevalmin - ekq[b] + p->freqevalstep
                               : p->freqevalmin - ekq[b] + iw * p->freqevalstep;
         Fact fc = (sem == FF_SEM_CPP && iw > 0)
                       ? fasx[(size_t)b * NE]
-                      : fact(sem, 1, true, wx, nF, grid.data(), sorted);
+                      : fact(sem, 1, true, wx, nF, grid.data(), sorted, &walk);
         const int kind = (sem == FF_SEM_F90 && wx < 0.0) ? kA : kR;
         fasx[(size_t)b * NE + iw] = fc;
         kasx[(size_t)b * NE + iw] = (signed char)kind;
@@ -232,11 +254,13 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
           mark(kind, fc.f + 1, b);
         }
       }
+    }
   // cor: [NB][NE]
   std::vector<Fact> fcor((size_t)NB * NE);
   std::vector<signed char> pcor((size_t)NB * NE, 0);
   if ((parts & FF_PART_COR) && sums_possible)
-    for (int b = 0; b < NB; ++b)
+    for (int b = 0; b < NB; ++b) {
+      Walker walk;
       for (int iw = 0; iw < NE; ++iw) {
         const double wx = sem == FF_SEM_CPP
                               ? p->freqevalmin - ekq[b] + p->freqevalstep
@@ -244,7 +268,7 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
         const bool occ = b < NV;
         Fact fc = (sem == FF_SEM_CPP && iw > 0)
                       ? fcor[(size_t)b * NE]
-                      : fact(sem, 2, occ, wx, nF, grid.data(), sorted);
+                      : fact(sem, 2, occ, wx, nF, grid.data(), sorted, &walk);
         fcor[(size_t)b * NE + iw] = fc;
         pcor[(size_t)b * NE + iw] =
             (signed char)(sem == FF_SEM_CPP ? wx > 0.0 : wx >= 0.0);
@@ -254,6 +278,7 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
             mark(kind, fc.f + 1, b);
           }
       }
+    }
 
   // ---- pack passes: up to 8 bands per (kind, slice)
   std::vector<SliceJob> jobs;
@@ -282,6 +307,8 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
       job_slice.push_back(si);
     }
   const int njobs = (int)jobs.size();
+  const double plan_ms =
+      std::chrono::duration<double, std::milli>(clk::now() - t_plan).count();
   if (njobs > 65535)
     return set_err(h, GPP_EUNSUPPORTED, "%d slice passes (max 65535)", njobs);
 
@@ -437,6 +464,7 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   HIP_OR_RET(h, hipStreamSynchronize(s));
 
   // ---- assembly
+  const auto t_asm = clk::now();
   auto T = [&](int kind, int f, int b) -> Cplx {
     const int slot = need[((size_t)kind * nF + f) * NB + b];
     Cplx c;
@@ -497,6 +525,9 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
       r->achDtemp_cor[iw] = gpp_c128{cor[iw].re + (q ? q[6] : 0.0),
                                      cor[iw].im + (q ? q[7] : 0.0)};
   }
+  r->host_plan_ms = plan_ms;
+  r->host_assemble_ms =
+      std::chrono::duration<double, std::milli>(clk::now() - t_asm).count();
   float ms = 0.f;
   HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
   r->slice_kernel_ms = ms;

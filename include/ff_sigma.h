@@ -91,6 +91,8 @@ typedef struct {
   long long slices;               /* distinct (array, frequency) slices    */
   long long slice_terms;          /* (n1, slice, G', G) products formed    */
   long long freq_terms;           /* (n1, ifreq, iw) grid terms            */
+  double host_plan_ms;            /* host: grid searches and pass packing  */
+  double host_assemble_ms;        /* host: interpolations of the sums      */
 } ff_result;
 
 void ff_default_options(ff_options *o);
