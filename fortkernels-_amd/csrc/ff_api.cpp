@@ -327,8 +327,12 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   const size_t o_jobs = a.take(jobs.size() * sizeof(SliceJob));
   const size_t o_rec = a.take((size_t)njobs * per_job * gpp::ff::kRecSlots * 8);
   const size_t o_T = a.take((size_t)njobs * gpp::ff::kRecSlots * 8);
-  const size_t o_part = a.take(do_ach ? (size_t)NB * NE * gpp::ff::kFreqSlots * 8 : 0);
+  const int fsplit = do_ach ? gpp::ff::freq_splits(NB, NE, nF) : 1;
+  const size_t o_part = a.take(
+      do_ach ? (size_t)fsplit * NB * NE * gpp::ff::kFreqSlots * 8 : 0);
   const size_t o_fout = a.take(do_ach ? (size_t)NE * gpp::ff::kFreqSlots * 8 : 0);
+  const size_t o_rows =
+      a.take(do_ach ? (size_t)NB * nF * gpp::ff::kRowDoubles * 8 : 0);
   // host inputs: small arrays and aqsm/aqsn staged here too
   const bool need_aqs = njobs > 0;
   size_t o_aqsm = 0, o_aqsn = 0, o_vc = 0, o_inv = 0, o_ind = 0, o_ek = 0,
@@ -445,9 +449,11 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
     fa.fmin = p->freqevalmin;
     fa.fstep = p->freqevalstep;
     fa.pref_zb = p->pref_zb;
-    HIP_OR_RET(h, gpp::ff::launch_freq(fa, sem, (double *)(base + o_part), s));
+    HIP_OR_RET(h, gpp::ff::launch_freq(fa, sem, fsplit,
+                                       (double *)(base + o_rows),
+                                       (double *)(base + o_part), s));
     HIP_OR_RET(h, gpp::ff::launch_freq_reduce((const double *)(base + o_part),
-                                              NB, NE,
+                                              fsplit * NB, NE,
                                               (double *)(base + o_fout), s));
   }
   HIP_OR_RET(h, hipEventRecord(h->ev[2], s));

@@ -17,7 +17,10 @@
 //
 // The frequency-grid terms (achDtemp_Kernel :155, ffKernel.f90:335-384)
 // touch no large array: one thread per (n1, iw) runs the grid.
+#include <algorithm>
+
 #include "ff_internal.h"
+#include "ff_math.h"
 
 namespace gpp {
 namespace ff {
@@ -173,72 +176,120 @@ __device__ __forceinline__ double clamp_ratio(double x) {
   return x;
 }
 
+// The per-(n1, f) values of the grid terms that do not depend on iw, formed
+// once (one thread each) with the operations of the reference's loop body
+// (ffKernel.f90:335-384, ffKernel.cpp:200-206); row layout:
+//   0,1  i pref[f] schDt[n1][f]        2  zb - Re brd        3  cor shift
+//   4    Im brd      5  right = zb     6  left = zb[f-1]     7  -
+//   8,9  i pref_zb avg (sch2 factor)   10,11 schDt[f-1]
+//   12,13 lin/(right-left)             14,15 lin = schDt[f] - schDt[f-1]
+__global__ void freq_prep_kernel(FreqArgs a, double *__restrict__ rows) {
+  const size_t n = (size_t)a.NB * a.nFreq;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int n1 = (int)(i / a.nFreq), f = (int)(i % a.nFreq);
+    const bool occ = n1 < a.NV;
+    const double zb = a.grid[f];
+    const double2 br = a.brd[f];
+    const double2 sdt = a.schdt[i];
+    double r[kRowDoubles];
+    r[0] = -a.pref[f] * sdt.y;
+    r[1] = a.pref[f] * sdt.x;
+    r[2] = zb - br.x;
+    r[3] = (occ ? -zb : zb) - br.x;
+    r[4] = br.y;
+    r[5] = zb;
+    r[7] = 0.0;
+    if (f > 0) {
+      const double2 sp = a.schdt[i - 1];
+      const double left = a.grid[f - 1];
+      const double2 avg =
+          make_double2(0.5 * (sdt.x + sp.x), 0.5 * (sdt.y + sp.y));
+      const double2 lin = make_double2(sdt.x - sp.x, sdt.y - sp.y);
+      const double rl = 1.0 / (zb - left);
+      r[6] = left;
+      r[8] = -a.pref_zb * avg.y;
+      r[9] = a.pref_zb * avg.x;
+      r[10] = sp.x;
+      r[11] = sp.y;
+      r[12] = lin.x * rl;
+      r[13] = lin.y * rl;
+      r[14] = lin.x;
+      r[15] = lin.y;
+    } else {
+      r[6] = 0.0;
+#pragma unroll
+      for (int k = 8; k < kRowDoubles; ++k) r[k] = 0.0;
+    }
+    double2 *o = (double2 *)(rows + i * kRowDoubles);
+#pragma unroll
+    for (int k = 0; k < kRowDoubles / 2; ++k)
+      o[k] = make_double2(r[2 * k], r[2 * k + 1]);
+  }
+}
+
 // One thread per (n1 = blockIdx.y, iw).  SEM 0 (ffKernel.cpp:200-206):
 //   ach -= (i pref[f] schDt[n1][f]) / (wx - coh[f]),  every f,
 //   wx = freqevalmin - ekq[n1] + (iw-1)*freqevalstep.
 // SEM 1 (ffKernel.f90:335-384), wx = freqevalmin - ekq[n1] + iw*freqevalstep:
 //   f != nFreq-1: ach  -= i pref schDt / (wx - coh); corb -= ... / (wx - cor)
 //   f != 0:       sch2 -= i pref_zb avg * (-log|ratio|) ; cor -= i pref_zb lin3
-// The grid values are wave-uniform (scalar loads).
+// A band's rows are wave-uniform (scalar loads).  blockIdx.z picks a
+// contiguous range of f (the grid is split so that small NE*NB still fills
+// the chip); part[((z*NB + n1)*NE + iw)*8].
 template <int SEM>
-__global__ __launch_bounds__(256) void freq_kernel(FreqArgs a,
+__global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
+                                                   const double *__restrict__ rows,
                                                    double *__restrict__ part) {
   const int n1 = blockIdx.y;
   const int iw = blockIdx.x * blockDim.x + threadIdx.x;
   if (iw >= a.NE) return;
+  const int f0 = blockIdx.z * fchunk;
+  const int f1 = min(a.nFreq, f0 + fchunk);
   const bool occ = n1 < a.NV;
   const double ek = a.ekq[n1];
   const double wx = SEM == 0 ? a.fmin - ek + (iw - 1) * a.fstep
                              : a.fmin - ek + iw * a.fstep;
-  const double2 *sd = a.schdt + (size_t)n1 * a.nFreq;
+  const double *rb = rows + (size_t)n1 * a.nFreq * kRowDoubles;
   double achr = 0.0, achi = 0.0, cbr = 0.0, cbi = 0.0;
   double s2r = 0.0, s2i = 0.0, cr = 0.0, ci = 0.0;
   // occupied lin3 argument: -freqevalmin - ekq + (iw-1)*freqevalstep (f90:375)
   const double wocc = -a.fmin - ek + iw * a.fstep;
-  double2 sd_prev = make_double2(0.0, 0.0);
-  for (int f = 0; f < a.nFreq; ++f) {
-    const double zb = a.grid[f];
-    const double2 br = a.brd[f];
-    const double2 sdt = sd[f];
-    const double2 num = make_double2(-a.pref[f] * sdt.y, a.pref[f] * sdt.x);
+  for (int f = f0; f < f1; ++f) {
+    const double *r = rb + (size_t)f * kRowDoubles;
+    const double2 num = make_double2(r[0], r[1]);
     if (SEM == 0 || f != a.nFreq - 1) {
-      const double2 t1 = cdiv(num, make_double2(wx - (zb - br.x), br.y));
+      const double2 t1 = cdiv(num, make_double2(wx - r[2], r[4]));
       achr -= t1.x;
       achi -= t1.y;
       if (SEM == 1) {
-        const double corr = (occ ? -zb : zb) - br.x;
-        const double2 t2 = cdiv(num, make_double2(wx - corr, br.y));
+        const double2 t2 = cdiv(num, make_double2(wx - r[3], r[4]));
         cbr -= t2.x;
         cbi -= t2.y;
       }
     }
     if (SEM == 1 && f != 0) {
-      const double right = zb, left = a.grid[f - 1];
-      const double2 avg =
-          make_double2(0.5 * (sdt.x + sd_prev.x), 0.5 * (sdt.y + sd_prev.y));
-      const double2 lin = make_double2(sdt.x - sd_prev.x, sdt.y - sd_prev.y);
-      const double rl = 1.0 / (right - left);
-      const double2 lin2 = make_double2(lin.x * rl, lin.y * rl);
-      double it = -log(clamp_ratio(fabs((wx - right) / (wx - left))));
+      const double right = r[5], left = r[6];
+      double it = -log_pos(clamp_ratio(fabs((wx - right) / (wx - left))));
       // sch2 -= i pref_zb * avg * it
-      s2r -= -a.pref_zb * avg.y * it;
-      s2i -= a.pref_zb * avg.x * it;
+      s2r -= r[8] * it;
+      s2i -= r[9] * it;
       double ar;
       if (occ) {
-        it = log(clamp_ratio(fabs((wx + right) / (wx + left))));
+        it = log_pos(clamp_ratio(fabs((wx + right) / (wx + left))));
         ar = wocc - left;
       } else {
         ar = wx - left;
       }
-      const double l3r = (sd_prev.x + lin2.x * ar) * it + lin.x;
-      const double l3i = (sd_prev.y + lin2.y * ar) * it + lin.y;
+      const double l3r = (r[10] + r[12] * ar) * it + r[14];
+      const double l3i = (r[11] + r[13] * ar) * it + r[15];
       // cor -= i pref_zb * lin3
       cr -= -a.pref_zb * l3i;
       ci -= a.pref_zb * l3r;
     }
-    sd_prev = sdt;
   }
-  double *o = part + ((size_t)n1 * a.NE + iw) * kFreqSlots;
+  double *o =
+      part + (((size_t)blockIdx.z * a.NB + n1) * a.NE + iw) * kFreqSlots;
   o[0] = achr;
   o[1] = achi;
   o[2] = cbr;
@@ -249,15 +300,15 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a,
   o[7] = ci;
 }
 
-__global__ void freq_reduce_kernel(const double *__restrict__ part, int NB,
+__global__ void freq_reduce_kernel(const double *__restrict__ part, int nrow,
                                    int NE, double *__restrict__ out) {
   const int iw = blockIdx.x * blockDim.x + threadIdx.x;
   if (iw >= NE) return;
   double s[kFreqSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int n1 = 0; n1 < NB; ++n1)
+  for (int r = 0; r < nrow; ++r)
 #pragma unroll
     for (int k = 0; k < kFreqSlots; ++k)
-      s[k] += part[((size_t)n1 * NE + iw) * kFreqSlots + k];
+      s[k] += part[((size_t)r * NE + iw) * kFreqSlots + k];
 #pragma unroll
   for (int k = 0; k < kFreqSlots; ++k) out[(size_t)iw * kFreqSlots + k] = s[k];
 }
@@ -298,22 +349,38 @@ hipError_t launch_slice_reduce(const double *rec, int njobs, int per_job,
   return hipGetLastError();
 }
 
-hipError_t launch_freq(const FreqArgs &a, int sem, double *part,
-                       hipStream_t s) {
-  if (a.NB <= 0 || a.NE <= 0) return hipSuccess;
-  if (a.NB > 65535) return hipErrorInvalidValue;
-  const dim3 grid((a.NE + 255) / 256, a.NB);
+int freq_splits(int NB, int NE, int nFreq) {
+  // about eight 64-lane waves per SIMD (256 CUs x 4 SIMDs) over the
+  // (split, n1, iw) threads, at least 16 grid points per split, at most 64
+  // splits
+  const long long threads = (long long)NB * (((long long)NE + 255) / 256 * 256);
+  if (threads <= 0 || nFreq <= 0) return 1;
+  long long sp = (256LL * 4 * 8 * 64) / threads;
+  sp = std::min<long long>(sp, std::min(64, nFreq / 16));
+  return (int)std::max<long long>(1, sp);
+}
+
+hipError_t launch_freq(const FreqArgs &a, int sem, int splits, double *rows,
+                       double *part, hipStream_t s) {
+  if (a.NB <= 0 || a.NE <= 0 || a.nFreq <= 0) return hipSuccess;
+  if (a.NB > 65535 || splits < 1 || splits > 65535) return hipErrorInvalidValue;
+  const size_t n = (size_t)a.NB * a.nFreq;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  freq_prep_kernel<<<(unsigned)blocks, 256, 0, s>>>(a, rows);
+  const int fchunk = (a.nFreq + splits - 1) / splits;
+  const dim3 grid((a.NE + 255) / 256, a.NB, splits);
   if (sem == 0)
-    freq_kernel<0><<<grid, 256, 0, s>>>(a, part);
+    freq_kernel<0><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
   else
-    freq_kernel<1><<<grid, 256, 0, s>>>(a, part);
+    freq_kernel<1><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
   return hipGetLastError();
 }
 
-hipError_t launch_freq_reduce(const double *part, int NB, int NE, double *out,
-                              hipStream_t s) {
+hipError_t launch_freq_reduce(const double *part, int nrow, int NE,
+                              double *out, hipStream_t s) {
   if (NE <= 0) return hipSuccess;
-  freq_reduce_kernel<<<(NE + 255) / 256, 256, 0, s>>>(part, NB, NE, out);
+  freq_reduce_kernel<<<(NE + 255) / 256, 256, 0, s>>>(part, nrow, NE, out);
   return hipGetLastError();
 }
 

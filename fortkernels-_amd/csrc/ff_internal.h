@@ -15,6 +15,7 @@ constexpr int kSliceThreads = 256;
 constexpr int kRecSlots = 2 * kMaxJobBands;  // complex per band per record
 constexpr int kFreqSlots = 8;     // ach, corb, sch2 (grid part), cor (grid
                                   // part): complex each
+constexpr int kRowDoubles = 16;   // per (n1, f) grid-term constants
 
 // One pass over one frequency slice of I_epsR or I_epsA for up to 8 bands:
 //   T[b] = sum_{G'} cG[band_b][G'] sum_G aqsn[band_b][G] * slice[G'][G]
@@ -48,12 +49,15 @@ hipError_t launch_slices(const SliceJob *jobs, int njobs, const double2 *cG,
 hipError_t launch_slice_reduce(const double *rec, int njobs, int per_job,
                                double *T, hipStream_t s);
 int slice_chunks(int NC);
-// part[(n1*NE + iw)*8]; sem 0 = ffKernel.cpp, 1 = ffKernel.f90.
-hipError_t launch_freq(const FreqArgs &a, int sem, double *part,
-                       hipStream_t s);
-// out[iw*8] = sum over n1 (in order) of part.
-hipError_t launch_freq_reduce(const double *part, int NB, int NE, double *out,
-                              hipStream_t s);
+// Number of frequency-grid ranges the grid kernel splits the f sum into.
+int freq_splits(int NB, int NE, int nFreq);
+// part[((split*NB + n1)*NE + iw)*8]; sem 0 = ffKernel.cpp, 1 = ffKernel.f90;
+// rows is scratch of NB*nFreq*kRowDoubles doubles.
+hipError_t launch_freq(const FreqArgs &a, int sem, int splits, double *rows,
+                       double *part, hipStream_t s);
+// out[iw*8] = sum over the nrow = splits*NB rows of part, in order.
+hipError_t launch_freq_reduce(const double *part, int nrow, int NE,
+                              double *out, hipStream_t s);
 
 // Synthetic I_eps slices on the device (config 1: constants; config 2: the
 // counter-based values of ff_fill_inputs).
