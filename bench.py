@@ -34,6 +34,23 @@ Other workloads (their own JSON line, no CPU baseline):
                  [2,16], nodes_per_group 20) in one batched launch sequence
                  captured in a hipGraph, tile autotuned, against running them
                  one by one with the FP64 path
+  ff_bw, ff_cb   the full-frequency (FF) kernels at the two example runs of
+                 FFKer_cori/ffKernel.f90:9-14 ("bandwidth bound" 8 2 10000
+                 600 400 200, "compute bound" 8 2 1000 60 20000 30000),
+                 seeded random inputs (FF generator config 2) generated in
+                 device memory; one step = one evaluation of every FF output
+                 (ff_sigma_ex, device inputs, host results).  `value` is the
+                 ffKernel.cpp semantics (the reference binary's); the
+                 ffKernel.f90 semantics is timed beside it.  roofline: the
+                 slice kernel against HBM (distinct I_eps slices x 16 B per
+                 entry) when it dominates, else the frequency-grid kernel
+                 against the FP64 VALU peak (15 flops per ffKernel.cpp grid
+                 term; ffKernel.f90 terms as FLOP_FF_F90 below).  The mini-app
+                 fills (config 1) are checked against their closed forms at
+                 the full shape.  cpu_baseline: the reference binary on a
+                 sample (fewer evaluation frequencies, and fewer grid points
+                 where the grid is large), each of its four kernel timers
+                 scaled by its trip count to the full shape.
 """
 import argparse
 import json
@@ -51,6 +68,21 @@ NB, NV, NC, NPG = 512, 2, 32768, 20
 FLOP_UNOCC, FLOP_OCC = 25.0, 45.0
 PEAK_FP64_TFLOPS = 78.6
 METRIC = "GPP sigma terms/s on 1 MI355X, mixed vs FP64 @<=1e-8 rel err; % VALU peak"
+
+FF_WORKLOADS = {
+    "ff_bw": dict(shape=(8, 2, 10000, 600, 400, 200),
+                  desc="FF sigma, ffKernel.f90 bandwidth-bound example 8 2 10000 600 400 200"),
+    "ff_cb": dict(shape=(8, 2, 1000, 60, 20000, 30000),
+                  desc="FF sigma, ffKernel.f90 compute-bound example 8 2 1000 60 20000 30000"),
+}
+FF_METRIC = "FF sigma evaluations/s on 1 MI355X (all outputs, ffKernel.cpp semantics)"
+FLOP_FF_CPP = 15.0  # per (n1, ifreq, iw): wx - coh 1, |d|^2 3, 1/x 1,
+                    # num*conj(d) 6, *r 2, accumulate 2
+# ffKernel.f90 per (n1, ifreq, iw): the two pole terms 2 x 15 for ifreq <
+# nFreq-1; for ifreq > 0 the ratio 3, log 1, sch2 4, lin3 and its
+# accumulate 12 (+ a second ratio, log and shift, 5, for occupied bands)
+FLOP_FF_F90 = dict(pole=30.0, interval=20.0, interval_occ_extra=5.0)
+PEAK_HBM_GBS = 8000.0
 
 WORKLOADS = {
     "canonical": dict(config=2, nb=512, nv=2, nc=32768, npg=20, gen="host",
@@ -298,6 +330,156 @@ def run_single(args, name):
     return line
 
 
+def ff_closed_form(sem, nb, nv, nc, ng, nf, ne):
+    """Outputs of the mini-app fills (FF generator config 1): aqsm = aqsn =
+    0.5+0.5i, I_epsR = 0.5+0.5i, I_epsA = 0.5-0.5i, vcoul = 1, identity
+    maps, ekq[n1] = -10 + n1, schDt = 0, freqevalmin 0, freqevalstep 0.5:
+    interpolation weights sum to 1 (asx) or -1/2 (cor), aqsn conj(aqsm) =
+    1/2, I_epsR - I_epsA = i."""
+    K = float(ng) * nc
+    n1 = np.arange(nb)[:, None]
+    iw = np.arange(ne)[None, :]
+    occ = n1 < nv
+    wx = 10.0 - n1 + (0.5 if sem == "cpp" else 0.5 * iw) + 0.0 * iw
+    up = (wx >= 0) if sem == "f90" else np.ones_like(wx, bool)
+    asx = np.where(occ, np.where(up, 0.25 * K * (1 + 1j), 0.25 * K * (1 - 1j)), 0)
+    pos = (wx > 0) if sem == "cpp" else (wx >= 0)
+    s = -0.25j * K
+    a2 = np.where(pos & ((sem == "f90") | ~occ), s, 0)
+    cor = np.where(pos & ~occ, s, 0) + np.where(~pos & occ, s, 0)
+    return (-0.125 * nb * K * (1 + 1j), asx.sum(0), cor.sum(0), a2.sum(0))
+
+
+def ff_cpu_baseline(args, shape):
+    """The reference ffKernel.cpp binary on a sample of the shape; its four
+    kernel timers (achsDtemp, asxDtemp, achDtemp, achDtemp_cor) scaled to
+    the full shape by their trip counts: 1, nfreqeval, nfreqeval*nFreq,
+    nfreqeval."""
+    exe = ROOT / "oracle" / "_ref" / "ffKer_cpp"
+    if not exe.exists():
+        return None
+    nb, nv, nc, ng, nf, ne = shape
+    cores = host_cores()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread",
+               OMP_PLACES="cores")
+    nf_s = min(nf, 8 if nf <= 1000 else 2000)
+    names = ("achsDtemp", "asxDtemp", "achDtemp", "achDtemp_cor")
+
+    def run(ne_s):
+        out = subprocess.run([str(exe)] + [str(x) for x in (nb, nv, nc, ng, nf_s, ne_s)],
+                             env=env, capture_output=True, text=True, timeout=300,
+                             check=True).stdout
+        return {k: float(re.search(r"\*+ %s Kernel time\s+\*+= ([0-9.eE+-]+)" % k,
+                                   out).group(1)) for k in names}
+
+    t = run(2)
+    per_iw = (t["asxDtemp"] + t["achDtemp"] + t["achDtemp_cor"]) / 2
+    ne_s = int(max(2, min(ne, round(args.cpu_seconds / max(per_iw, 1e-6)))))
+    t = run(ne_s)
+    est = (t["achsDtemp"] + (t["asxDtemp"] + t["achDtemp_cor"]) * ne / ne_s +
+           t["achDtemp"] * (ne * nf) / (ne_s * nf_s))
+    return {"value": 1.0 / est, "unit": "evals/s", "cores": cores, "kind": "reference",
+            "sample": f"FFKer_cori/ffKernel.cpp binary (its constant fills) "
+                      f"{nb} {nv} {nc} {ng} {nf_s} {ne_s}, OMP_NUM_THREADS={cores}; "
+                      f"kernel timers " + ", ".join(f"{k} {t[k]:.3f} s" for k in names) +
+                      f"; scaled to {' '.join(map(str, shape))}: {est:.2f} s per evaluation"}
+
+
+def run_ff(args, name):
+    import torch
+    gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
+    w = FF_WORKLOADS[name]
+    nb, nv, nc, ng, nf, ne = shape = w["shape"]
+    stream = torch.cuda.current_stream()
+    h = gppk.Handle(0)
+    steps = max(1, args.steps)
+    runs, errs = {}, {}
+    for sem in ("cpp", "f90"):
+        # closed forms of the mini-app fills at the full shape
+        if not args.no_check:
+            d = gppk.ff_fill_inputs_device(1, *shape, semantics=sem)
+            torch.cuda.synchronize()
+            r = h.ff(d, semantics=sem, memory=gppk.MEM_DEVICE, stream=stream.cuda_stream)
+            achs, asx, cor, a2 = ff_closed_form(sem, *shape)
+            e = max(abs(r["achsDtemp"] - achs) / abs(achs),
+                    *(float(np.max(np.abs(r[k] - v)) / np.max(np.abs(v)))
+                      for k, v in (("asxDtemp", asx), ("achDtemp_cor", cor),
+                                   ("ach2Dtemp", a2))))
+            errs[sem] = e
+            log(f"{name} {sem}: mini-app fills vs closed form, max rel err {e:.3e}")
+            del d
+            torch.cuda.empty_cache()
+        d = gppk.ff_fill_inputs_device(2, *shape, semantics=sem, seed=args.seed)
+        torch.cuda.synchronize()
+        call = lambda: h.ff(d, semantics=sem, memory=gppk.MEM_DEVICE,
+                            stream=stream.cuda_stream)
+        for _ in range(max(1, args.warmup)):
+            call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = [call() for _ in range(steps)]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        r = res[-1]
+        mean = lambda k: float(np.mean([x[k] for x in res]))
+        runs[sem] = dict(ms_per_step=dt * 1e3, evals_per_s=1.0 / dt,
+                         slice_kernel_ms=mean("slice_kernel_ms"),
+                         freq_kernel_ms=mean("freq_kernel_ms"),
+                         host_plan_ms=mean("host_plan_ms"),
+                         host_assemble_ms=mean("host_assemble_ms"),
+                         slices=int(r["slices"]), slice_bytes=float(r["slice_bytes"]),
+                         slice_terms=int(r["slice_terms"]), freq_terms=int(r["freq_terms"]))
+        log(f"{name} {sem}: {dt*1e3:.3f} ms/evaluation, slice kernel "
+            f"{runs[sem]['slice_kernel_ms']:.3f} ms ({r['slices']} slices, "
+            f"{r['slice_bytes']/1e9:.2f} GB), grid kernel {runs[sem]['freq_kernel_ms']:.3f} ms, "
+            f"host plan {runs[sem]['host_plan_ms']:.2f} ms, assembly "
+            f"{runs[sem]['host_assemble_ms']:.2f} ms")
+        del d, res
+        torch.cuda.empty_cache()
+
+    def roofline(sem):
+        r = runs[sem]
+        per_slice = float(ng) * nc * 16
+        if r["slice_kernel_ms"] >= r["freq_kernel_ms"]:
+            byts = r["slices"] * per_slice
+            ach = byts / (r["slice_kernel_ms"] * 1e-3) / 1e9
+            return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "ff slice sums",
+                    "bytes_per_launch": byts, "avg_launch_ms": r["slice_kernel_ms"],
+                    "note": "algorithmic bytes = distinct I_eps slices x NG x NC x 16"}
+        if sem == "cpp":
+            fl = FLOP_FF_CPP * nb * nf * ne
+        else:
+            f = FLOP_FF_F90
+            fl = (f["pole"] * nb + f["interval"] * nb + f["interval_occ_extra"] * nv) * (nf - 1) * ne
+        ach = fl / (r["freq_kernel_ms"] * 1e-3) / 1e12
+        return {"bound": "valu", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
+                "kernel": "ff frequency-grid terms (constants, grid, reduction)",
+                "flops_per_launch": fl, "avg_launch_ms": r["freq_kernel_ms"],
+                "note": "division and logarithm counted as 1 flop each"}
+
+    line = {
+        "metric": FF_METRIC, "value": runs["cpp"]["evals_per_s"], "unit": "evals/s",
+        "n_gpus": 1, "steps": steps, "warmup": max(1, args.warmup),
+        "ms_per_step": runs["cpp"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": w["desc"], "number_bands": nb, "nvband": nv, "ncouls": nc,
+                   "ngpown": ng, "nFreq": nf, "nfreqeval": ne, "semantics": "cpp"},
+        "roofline": roofline("cpp"),
+        "paths": {sem: dict(r, roofline=roofline(sem),
+                            closed_form_max_rel_err=errs.get(sem))
+                  for sem, r in runs.items()},
+    }
+    if not args.no_cpu:
+        log("timing the reference FF binary")
+        cb = ff_cpu_baseline(args, shape)
+        if cb is not None:
+            line["cpu_baseline"] = cb
+    h.close()
+    return line
+
+
 def run_batch(args):
     import torch
     gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
@@ -377,7 +559,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="canonical",
-                    choices=list(WORKLOADS) + ["batch"])
+                    choices=list(WORKLOADS) + ["batch"] + list(FF_WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -410,7 +592,12 @@ def main():
               "(the sum would shard over G' columns with one all-reduce of 12 "
               "doubles); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    line = run_batch(args) if args.workload == "batch" else run_single(args, args.workload)
+    if args.workload == "batch":
+        line = run_batch(args)
+    elif args.workload in FF_WORKLOADS:
+        line = run_ff(args, args.workload)
+    else:
+        line = run_single(args, args.workload)
     print(json.dumps(line), flush=True)
 
 

@@ -143,30 +143,33 @@ __global__ __launch_bounds__(kSliceThreads) void slice_kernel(
 }
 
 constexpr int kRedThreads = 256;
+constexpr int kRedJobsPerBlock = kRedThreads / kWave;
+static_assert(kRecSlots == 16 && kFreqSlots == 8,
+              "the reduction shuffles assume 16 record and 8 grid slots");
 
-// One block per job: fixed-order (strided Neumaier, then a tree) sum of its
-// records, slot by slot.
+// One wave per job: lane = (sub-stream lane>>4, slot lane&15).  Each lane
+// sums its sub-stream of the job's records (records sub, sub+4, ...) with
+// Neumaier compensation; the four sub-streams are then added in a fixed
+// order (xor 16, then xor 32).
 __global__ __launch_bounds__(kRedThreads) void slice_reduce_kernel(
-    const double *__restrict__ rec, int per_job, double *__restrict__ T) {
-  __shared__ double buf[kRedThreads];
-  const double *r = rec + (size_t)blockIdx.x * per_job * kRecSlots;
-  for (int s = 0; s < kRecSlots; ++s) {
-    double acc = 0.0, comp = 0.0;
-    for (int i = threadIdx.x; i < per_job; i += kRedThreads) {
-      const double x = r[(size_t)i * kRecSlots + s];
-      const double t = acc + x;
-      comp += (fabs(acc) >= fabs(x)) ? ((acc - t) + x) : ((x - t) + acc);
-      acc = t;
-    }
-    buf[threadIdx.x] = acc + comp;
-    __syncthreads();
-    for (int h = kRedThreads / 2; h > 0; h >>= 1) {
-      if (threadIdx.x < h) buf[threadIdx.x] += buf[threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) T[(size_t)blockIdx.x * kRecSlots + s] = buf[0];
-    __syncthreads();
+    const double *__restrict__ rec, int njobs, int per_job,
+    double *__restrict__ T) {
+  const int job = blockIdx.x * kRedJobsPerBlock + threadIdx.x / kWave;
+  if (job >= njobs) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int slot = lane & (kRecSlots - 1), sub = lane / kRecSlots;
+  const double *r = rec + (size_t)job * per_job * kRecSlots;
+  double acc = 0.0, comp = 0.0;
+  for (int i = sub; i < per_job; i += kWave / kRecSlots) {
+    const double x = r[(size_t)i * kRecSlots + slot];
+    const double t = acc + x;
+    comp += (fabs(acc) >= fabs(x)) ? ((acc - t) + x) : ((x - t) + acc);
+    acc = t;
   }
+  double v = acc + comp;
+  v += __shfl_xor(v, 16, kWave);
+  v += __shfl_xor(v, 32, kWave);
+  if (lane < kRecSlots) T[(size_t)job * kRecSlots + slot] = v;
 }
 
 // |x| clamped to [1e-4, 1e4] as ffKernel.f90:369-371 (NaN passes through).
@@ -300,17 +303,22 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
   o[7] = ci;
 }
 
-__global__ void freq_reduce_kernel(const double *__restrict__ part, int nrow,
-                                   int NE, double *__restrict__ out) {
-  const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per iw: lane = (row stream lane>>3, slot lane&7); each lane sums
+// rows sub, sub+8, ... of part in order, then the eight streams are added
+// in a fixed order (xor 8, 16, 32).
+__global__ __launch_bounds__(256) void freq_reduce_kernel(
+    const double *__restrict__ part, int nrow, int NE, double *__restrict__ out) {
+  const int iw = blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
   if (iw >= NE) return;
-  double s[kFreqSlots] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int r = 0; r < nrow; ++r)
-#pragma unroll
-    for (int k = 0; k < kFreqSlots; ++k)
-      s[k] += part[((size_t)r * NE + iw) * kFreqSlots + k];
-#pragma unroll
-  for (int k = 0; k < kFreqSlots; ++k) out[(size_t)iw * kFreqSlots + k] = s[k];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int slot = lane & (kFreqSlots - 1), sub = lane / kFreqSlots;
+  double v = 0.0;
+  for (int r = sub; r < nrow; r += kWave / kFreqSlots)
+    v += part[((size_t)r * NE + iw) * kFreqSlots + slot];
+  v += __shfl_xor(v, 8, kWave);
+  v += __shfl_xor(v, 16, kWave);
+  v += __shfl_xor(v, 32, kWave);
+  if (lane < kFreqSlots) out[(size_t)iw * kFreqSlots + slot] = v;
 }
 
 }  // namespace
@@ -345,7 +353,8 @@ hipError_t launch_slices(const SliceJob *jobs, int njobs, const double2 *cG,
 hipError_t launch_slice_reduce(const double *rec, int njobs, int per_job,
                                double *T, hipStream_t s) {
   if (njobs <= 0) return hipSuccess;
-  slice_reduce_kernel<<<njobs, kRedThreads, 0, s>>>(rec, per_job, T);
+  slice_reduce_kernel<<<(njobs + kRedJobsPerBlock - 1) / kRedJobsPerBlock,
+                        kRedThreads, 0, s>>>(rec, njobs, per_job, T);
   return hipGetLastError();
 }
 
@@ -380,7 +389,7 @@ hipError_t launch_freq(const FreqArgs &a, int sem, int splits, double *rows,
 hipError_t launch_freq_reduce(const double *part, int nrow, int NE,
                               double *out, hipStream_t s) {
   if (NE <= 0) return hipSuccess;
-  freq_reduce_kernel<<<(NE + 255) / 256, 256, 0, s>>>(part, nrow, NE, out);
+  freq_reduce_kernel<<<(NE + 3) / 4, 256, 0, s>>>(part, nrow, NE, out);
   return hipGetLastError();
 }
 

@@ -193,3 +193,48 @@ def test_cli_usage():
     out = subprocess.run([str(gppk.FF_CLI_PATH)], capture_output=True, text=True)
     assert out.returncode == 0
     assert "number_bands nvband ncouls ngpown nFreq nfreqeval" in out.stdout
+
+
+LOG_CHECK = r"""
+#include "ff_math.h"
+#include <cmath>
+#include <cstdio>
+#include <random>
+static double ulps(double got, long double want) {
+  const double w = (double)want;
+  const double u = std::nextafter(std::fabs(w), INFINITY) - std::fabs(w);
+  return (double)(std::fabs((long double)got - want) / u);
+}
+int main() {
+  std::mt19937_64 g(7);
+  std::uniform_real_distribution<double> wide(std::log(1e-4), std::log(1e4));
+  std::uniform_real_distribution<double> near(-1.0, 1.0);
+  double w = 0, n = 0;
+  for (int i = 0; i < 2000000; ++i) {
+    const double x = std::exp(wide(g));
+    w = std::fmax(w, ulps(gpp::ff::log_pos(x), logl((long double)x)));
+    const double y = 1.0 + near(g) * std::pow(10.0, -1 - i % 14);
+    if (y != 1.0) n = std::fmax(n, ulps(gpp::ff::log_pos(y), logl((long double)y)));
+  }
+  const double e1 = gpp::ff::log_pos(1.0), e2 = gpp::ff::log_pos(NAN);
+  std::printf("%.6f %.6f %g %d\n", w, n, e1, (int)std::isnan(e2));
+}
+"""
+
+
+def test_grid_kernel_logarithm_accuracy(tmp_path):
+    """The logarithm of the frequency-grid kernel (csrc/ff_math.h, built for
+    the host with g++) against long double logl: at most 1.25 ulp over the
+    clamped range [1e-4, 1e4], at most 0.6 ulp within 0.1 of 1."""
+    import shutil
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "log_check.cpp"
+    src.write_text(LOG_CHECK)
+    exe = tmp_path / "log_check"
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-I",
+                    str(gppk.PKG_DIR / "csrc"), str(src), "-o", str(exe)], check=True)
+    w, n, one, nan = subprocess.run([str(exe)], capture_output=True, text=True,
+                                    check=True, timeout=120).stdout.split()
+    assert float(w) <= 1.25 and float(n) <= 0.6, (w, n)
+    assert float(one) == 0.0 and nan == "1"
