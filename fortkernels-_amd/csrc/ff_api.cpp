@@ -323,7 +323,14 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   const int per_job = ngroups * nchunk;
   const bool do_ach = (parts & FF_PART_ACH) && NB > 0 && NE > 0;
   Arena a;
-  const size_t o_cg = a.take((size_t)NB * NG * 16);
+  // coefficient table for up to jbatch passes at a time (at most 1 GiB)
+  const int jbatch =
+      njobs > 0 ? (int)std::max<size_t>(
+                      1, std::min<size_t>((size_t)njobs,
+                                          ((size_t)1 << 30) /
+                                              ((size_t)NG * gpp::ff::kJobCgBytes)))
+                : 0;
+  const size_t o_cg = a.take((size_t)jbatch * NG * gpp::ff::kJobCgBytes);
   const size_t o_jobs = a.take(jobs.size() * sizeof(SliceJob));
   const size_t o_rec = a.take((size_t)njobs * per_job * gpp::ff::kRecSlots * 8);
   const size_t o_T = a.take((size_t)njobs * gpp::ff::kRecSlots * 8);
@@ -431,11 +438,17 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   HIP_OR_RET(h, hipEventRecord(h->ev[0], s));
   if (njobs > 0) {
     double2 *cG = (double2 *)(base + o_cg);
-    HIP_OR_RET(h, gpp::ff::launch_cg(aqsm, vcoul, inv, indinv, NB, NC, NG, cG,
-                                     s));
-    HIP_OR_RET(h, gpp::ff::launch_slices((const SliceJob *)(base + o_jobs),
-                                         njobs, cG, aqsn, NC, NG, ngroups,
-                                         (double *)(base + o_rec), s));
+    const SliceJob *djobs = (const SliceJob *)(base + o_jobs);
+    for (int j0 = 0; j0 < njobs; j0 += jbatch) {
+      const int nj = std::min(jbatch, njobs - j0);
+      HIP_OR_RET(h, gpp::ff::launch_cg(djobs + j0, nj, aqsm, vcoul, inv,
+                                       indinv, NC, NG, cG, s));
+      HIP_OR_RET(h, gpp::ff::launch_slices(
+                        djobs + j0, nj, cG, aqsn, NC, NG, ngroups,
+                        (double *)(base + o_rec) +
+                            (size_t)j0 * per_job * gpp::ff::kRecSlots,
+                        s));
+    }
     HIP_OR_RET(h, gpp::ff::launch_slice_reduce((const double *)(base + o_rec),
                                                njobs, per_job,
                                                (double *)(base + o_T), s));

@@ -57,62 +57,71 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
   return make_double2((a.x * b.x + a.y * b.y) * r, (a.y * b.x - a.x * b.y) * r);
 }
 
-__global__ void cg_kernel(const double2 *__restrict__ aqsm,
-                          const double *__restrict__ vcoul,
-                          const int *__restrict__ inv,
-                          const int *__restrict__ indinv, int NB, int NC,
-                          int NG, double2 *__restrict__ cG) {
-  const size_t n = (size_t)NB * NG;
+// Coefficient table of a run of passes: for pass j (jobs[j]), row G' and
+// slot b < 8,
+//   cGj[(j*NG + G')*8 + b] = vcoul[igp] conj(aqsm[band_b][igp])
+// with igp = indinv[inv_igp_index[G']], and 0 for skipped rows (igp outside
+// [0, NC)) and for slots b >= nb.  One pass's row is 128 contiguous bytes:
+// two scalar loads in the slice kernel.
+__global__ void job_cg_kernel(const SliceJob *__restrict__ jobs, int njobs,
+                              const double2 *__restrict__ aqsm,
+                              const double *__restrict__ vcoul,
+                              const int *__restrict__ inv,
+                              const int *__restrict__ indinv, int NC, int NG,
+                              double2 *__restrict__ cGj) {
+  const size_t n = (size_t)njobs * NG * kMaxJobBands;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    const int b = (int)(i / NG), g = (int)(i % NG);
-    const int indigp = inv[g];
-    int igp = (indigp >= 0 && indigp < NC) ? indinv[indigp] : -1;
-    if (igp >= NC) igp = -1;
-    if (igp < 0) {
-      cG[i] = make_double2(0.0, 0.0);
-    } else {
-      const double2 a = aqsm[(size_t)b * NC + igp];
-      const double v = vcoul[igp];
-      cG[i] = make_double2(v * a.x, -v * a.y);
+    const int b = (int)(i % kMaxJobBands);
+    const size_t jr = i / kMaxJobBands;
+    const int g = (int)(jr % NG), j = (int)(jr / NG);
+    double2 v = make_double2(0.0, 0.0);
+    if (b < jobs[j].nb) {
+      const int indigp = inv[g];
+      int igp = (indigp >= 0 && indigp < NC) ? indinv[indigp] : -1;
+      if (igp >= NC) igp = -1;
+      if (igp >= 0) {
+        const double2 a = aqsm[(size_t)jobs[j].band[b] * NC + igp];
+        const double vc = vcoul[igp];
+        v = make_double2(vc * a.x, -vc * a.y);
+      }
     }
+    cGj[i] = v;
   }
 }
 
-// Block: job blockIdx.y, row group g and G chunk c from blockIdx.x.  Each
+// Block: pass blockIdx.y, row group g and G chunk c from blockIdx.x.  Each
 // thread owns one G and runs the group's G' rows:
-//   W[b] += cG[band_b][G'] * slice[G'][G]     (cG wave-uniform: scalar loads)
-// then forms aqsn[band_b][G] * W[b], summed over the block into one record.
+//   W[b] += cGj[pass][G'][b] * slice[G'][G]   (wave-uniform: scalar loads)
+// for all 8 slots (the unused ones have zero coefficients), then forms
+// aqsn[band_b][G] * W[b], summed over the block into one record.
 __global__ __launch_bounds__(kSliceThreads) void slice_kernel(
-    const SliceJob *__restrict__ jobs, const double2 *__restrict__ cG,
+    const SliceJob *__restrict__ jobs, const double2 *__restrict__ cGj,
     const double2 *__restrict__ aqsn, int NC, int NG, int ngroups,
     int nchunk, double *__restrict__ rec) {
   __shared__ double red[kSliceThreads / kWave][kRecSlots];
   const SliceJob &job = jobs[blockIdx.y];
   const int nb = job.nb;
   const GlobalPtr<double2> sl = (GlobalPtr<double2>)job.slice;
+  const double2 *cg = cGj + (size_t)blockIdx.y * NG * kMaxJobBands;
   const int c = blockIdx.x % nchunk, grp = blockIdx.x / nchunk;
   const int rows = (NG + ngroups - 1) / ngroups;
   const int r0 = grp * rows, r1 = min(NG, r0 + rows);
   const int ig = c * kSliceThreads + threadIdx.x;
   const bool live = ig < NC;
   const int igc = live ? ig : NC - 1;
-  int band[kMaxJobBands];
-#pragma unroll
-  for (int b = 0; b < kMaxJobBands; ++b) band[b] = job.band[b];
   double wr[kMaxJobBands], wi[kMaxJobBands];
 #pragma unroll
   for (int b = 0; b < kMaxJobBands; ++b) wr[b] = wi[b] = 0.0;
-#pragma unroll 2
+#pragma unroll 4
   for (int row = r0; row < r1; ++row) {
     const double2 y = ldg(sl + (size_t)row * NC + igc);
+    const double2 *cr = cg + (size_t)row * kMaxJobBands;
 #pragma unroll
     for (int b = 0; b < kMaxJobBands; ++b) {
-      if (b < nb) {
-        const double2 cg = cG[(size_t)band[b] * NG + row];
-        wr[b] = fma(cg.x, y.x, fma(-cg.y, y.y, wr[b]));
-        wi[b] = fma(cg.x, y.y, fma(cg.y, y.x, wi[b]));
-      }
+      const double2 k = cr[b];
+      wr[b] = fma(k.x, y.x, fma(-k.y, y.y, wr[b]));
+      wi[b] = fma(k.x, y.y, fma(k.y, y.x, wi[b]));
     }
   }
   double v[kRecSlots];
@@ -120,7 +129,7 @@ __global__ __launch_bounds__(kSliceThreads) void slice_kernel(
   for (int b = 0; b < kMaxJobBands; ++b) {
     double pr = 0.0, pi = 0.0;
     if (b < nb && live) {
-      const double2 a = aqsn[(size_t)band[b] * NC + ig];
+      const double2 a = aqsn[(size_t)job.band[b] * NC + ig];
       pr = a.x * wr[b] - a.y * wi[b];
       pi = a.x * wi[b] + a.y * wr[b];
     }
@@ -325,19 +334,20 @@ __global__ __launch_bounds__(256) void freq_reduce_kernel(
 
 int slice_chunks(int NC) { return (NC + kSliceThreads - 1) / kSliceThreads; }
 
-hipError_t launch_cg(const double2 *aqsm, const double *vcoul,
-                     const int *inv_igp_index, const int *indinv, int NB,
-                     int NC, int NG, double2 *cG, hipStream_t s) {
-  const size_t n = (size_t)NB * NG;
+hipError_t launch_cg(const SliceJob *jobs, int njobs, const double2 *aqsm,
+                     const double *vcoul, const int *inv_igp_index,
+                     const int *indinv, int NC, int NG, double2 *cGj,
+                     hipStream_t s) {
+  const size_t n = (size_t)njobs * NG * kMaxJobBands;
   if (n == 0) return hipSuccess;
   size_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  cg_kernel<<<(unsigned)blocks, 256, 0, s>>>(aqsm, vcoul, inv_igp_index,
-                                             indinv, NB, NC, NG, cG);
+  job_cg_kernel<<<(unsigned)blocks, 256, 0, s>>>(
+      jobs, njobs, aqsm, vcoul, inv_igp_index, indinv, NC, NG, cGj);
   return hipGetLastError();
 }
 
-hipError_t launch_slices(const SliceJob *jobs, int njobs, const double2 *cG,
+hipError_t launch_slices(const SliceJob *jobs, int njobs, const double2 *cGj,
                          const double2 *aqsn, int NC, int NG, int ngroups,
                          double *rec, hipStream_t s) {
   if (njobs <= 0 || NG <= 0) return hipSuccess;
@@ -346,7 +356,7 @@ hipError_t launch_slices(const SliceJob *jobs, int njobs, const double2 *cG,
   const long long bx = (long long)nchunk * ngroups;
   if (bx > 0x7fffffffLL) return hipErrorInvalidValue;
   slice_kernel<<<dim3((unsigned)bx, (unsigned)njobs), kSliceThreads, 0, s>>>(
-      jobs, cG, aqsn, NC, NG, ngroups, nchunk, rec);
+      jobs, cGj, aqsn, NC, NG, ngroups, nchunk, rec);
   return hipGetLastError();
 }
 

@@ -13,6 +13,7 @@ namespace ff {
 constexpr int kMaxJobBands = 8;   // bands sharing one pass over a slice
 constexpr int kSliceThreads = 256;
 constexpr int kRecSlots = 2 * kMaxJobBands;  // complex per band per record
+constexpr size_t kJobCgBytes = 16 * kMaxJobBands;  // per pass and G' row
 constexpr int kFreqSlots = 8;     // ach, corb, sch2 (grid part), cor (grid
                                   // part): complex each
 constexpr int kRowDoubles = 16;   // per (n1, f) grid-term constants
@@ -20,6 +21,7 @@ constexpr int kRowDoubles = 16;   // per (n1, f) grid-term constants
 // One pass over one frequency slice of I_epsR or I_epsA for up to 8 bands:
 //   T[b] = sum_{G'} cG[band_b][G'] sum_G aqsn[band_b][G] * slice[G'][G]
 // with cG[n1][G'] = vcoul[igp] * conj(aqsm[n1][igp]) (0 for skipped rows).
+// Scratch per pass: NG x 8 coefficients (kJobCgBytes per row).
 struct SliceJob {
   const double2 *slice;  // [NG][NC]
   int nb;
@@ -38,11 +40,15 @@ struct FreqArgs {
   double fmin, fstep, pref_zb;
 };
 
-hipError_t launch_cg(const double2 *aqsm, const double *vcoul,
-                     const int *inv_igp_index, const int *indinv, int NB,
-                     int NC, int NG, double2 *cG, hipStream_t s);
-// Grid: jobs x row groups x G chunks; rec[(job*ngroups + g)*nchunk + c][16].
-hipError_t launch_slices(const SliceJob *jobs, int njobs, const double2 *cG,
+// cGj[(j*NG + G')*8 + b] = vcoul[igp] conj(aqsm[band_b][igp]) for the
+// passes jobs[0, njobs) (0 for skipped rows and unused slots).
+hipError_t launch_cg(const SliceJob *jobs, int njobs, const double2 *aqsm,
+                     const double *vcoul, const int *inv_igp_index,
+                     const int *indinv, int NC, int NG, double2 *cGj,
+                     hipStream_t s);
+// Grid: jobs x row groups x G chunks; rec[(job*ngroups + g)*nchunk + c][16];
+// cGj as launch_cg wrote it for the same jobs.
+hipError_t launch_slices(const SliceJob *jobs, int njobs, const double2 *cGj,
                          const double2 *aqsn, int NC, int NG, int ngroups,
                          double *rec, hipStream_t s);
 // T[job][16] = fixed-order sum of the job's ngroups*nchunk records.
