@@ -443,8 +443,12 @@ def run_ff(args, name):
         if r["slice_kernel_ms"] >= r["freq_kernel_ms"]:
             byts = r["slices"] * per_slice
             ach = byts / (r["slice_kernel_ms"] * 1e-3) / 1e9
+            traffic = None
+            tf = ROOT / "profiles" / f"traffic_ff_slice_{'_'.join(map(str, shape))}.json"
+            if tf.exists():
+                traffic = json.loads(tf.read_text())["hbm_bytes_per_launch"].get(sem)
             return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": "ff slice sums",
+                    "frac": ach / PEAK_HBM_GBS, "traffic": traffic, "kernel": "ff slice sums",
                     "bytes_per_launch": byts, "avg_launch_ms": r["slice_kernel_ms"],
                     "note": "algorithmic bytes = distinct I_eps slices x NG x NC x 16"}
         if sem == "cpp":

@@ -275,7 +275,9 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
       achr -= t1.x;
       achi -= t1.y;
       if (SEM == 1) {
-        const double2 t2 = cdiv(num, make_double2(wx - r[3], r[4]));
+        // unoccupied bands: the cor pole is the coh pole (row 3 == row 2)
+        const double2 t2 =
+            occ ? cdiv(num, make_double2(wx - r[3], r[4])) : t1;
         cbr -= t2.x;
         cbi -= t2.y;
       }
@@ -369,13 +371,16 @@ hipError_t launch_slice_reduce(const double *rec, int njobs, int per_job,
 }
 
 int freq_splits(int NB, int NE, int nFreq) {
-  // about eight 64-lane waves per SIMD (256 CUs x 4 SIMDs) over the
-  // (split, n1, iw) threads, at least 16 grid points per split, at most 64
-  // splits
+  // Every (split, n1, iw) thread runs the same number of grid points, so the
+  // busiest SIMD sets the time: aim for about 64 waves per SIMD (256 CUs x 4
+  // SIMDs), which keeps the imbalance near 1/64; at least 16 grid points per
+  // split, at most 64 splits and 256 MiB of partial sums.
   const long long threads = (long long)NB * (((long long)NE + 255) / 256 * 256);
   if (threads <= 0 || nFreq <= 0) return 1;
-  long long sp = (256LL * 4 * 8 * 64) / threads;
+  long long sp = (256LL * 4 * 64 * 64) / threads;
   sp = std::min<long long>(sp, std::min(64, nFreq / 16));
+  const long long part_bytes = (long long)NB * NE * kFreqSlots * 8;
+  if (part_bytes > 0) sp = std::min<long long>(sp, (256LL << 20) / part_bytes);
   return (int)std::max<long long>(1, sp);
 }
 
