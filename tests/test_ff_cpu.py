@@ -63,13 +63,22 @@ def test_kat_inputs_exercise_both_cor_branches():
         assert (wx > 0).any() and (wx <= 0).any()
 
 
-def closed_form(sem, nb, nv, nc, ng, nf, ne):
-    """The mini-app's fills (aqsm = aqsn = 0.5+0.5i, I_epsR = 0.5+0.5i,
-    I_epsA = 0.5-0.5i, vcoul = 1, identity maps, ekq = -10+n1, schDt = 0):
-    every interpolation has weights summing to 1 (asx) or -1/2 (cor, wx > 0),
-    aqsn*conj(aqsm) = 1/2, I_epsR - I_epsA = i."""
+def closed_form(sem, nb, nv, nc, ng, nf, ne, beta=0.0, gamma=0.0):
+    """The mini-app's fills (aqsm = aqsn = 0.5+0.5i, vcoul = 1, identity
+    maps, ekq = -10+n1, schDt = 0, grid g_f = 2f) with I_eps linear in the
+    grid frequency and constant over (G', G):
+        I_epsR[f] = 0.5+0.5i + beta g_f,   I_epsA[f] = 0.5-0.5i + gamma g_f
+    (beta = gamma = 0: the fills themselves).  Every interpolation of both
+    sources evaluates the linear interpolant at x = |wx| with weights that
+    reproduce a linear function exactly, inside the grid or extrapolated
+    from the interval a search falls back to; the cor/ach2 weights carry a
+    factor -1/2.  With aqsn*conj(aqsm) = 1/2 each sum is K/2 times the
+    interpolated value, K = ng*nc."""
     K = ng * nc
-    achs = -0.125 * nb * K * (1 + 1j)
+    IR = lambda x: 0.5 + 0.5j + beta * x
+    IA = lambda x: 0.5 - 0.5j + gamma * x
+    g_achs = 2.0 * (1 if sem == "cpp" else 0)  # cpp: slice 1, f90: slice 0
+    achs = -0.5 * nb * 0.5 * K * IR(g_achs)
     asx = np.zeros(ne, complex)
     cor = np.zeros(ne, complex)
     a2 = np.zeros(ne, complex)
@@ -77,13 +86,12 @@ def closed_form(sem, nb, nv, nc, ng, nf, ne):
         occ = n1 < nv
         for iw in range(ne):
             wx = 0.0 - (-10.0 + n1) + (0.5 if sem == "cpp" else iw * 0.5)
+            x = abs(wx)
             if occ:
-                if sem == "cpp" or wx >= 0:
-                    asx[iw] += 0.25 * K * (1 + 1j)
-                else:
-                    asx[iw] += 0.25 * K * (1 - 1j)
+                src = IR if (sem == "cpp" or wx >= 0) else IA
+                asx[iw] += 0.5 * K * src(x)
             pos = wx > 0 if sem == "cpp" else wx >= 0
-            s = -0.25j * K
+            s = 0.5 * K * -0.5 * (IR(x) - IA(x))
             if pos:
                 if sem == "f90" or not occ:
                     a2[iw] += s
@@ -92,6 +100,18 @@ def closed_form(sem, nb, nv, nc, ng, nf, ne):
             elif occ:
                 cor[iw] += s
     return achs, asx, cor, a2
+
+
+def linear_eps(inp, beta, gamma):
+    """Config-1 inputs with I_eps linear in the grid frequency."""
+    g = inp["dFreqGrid"][:, None, None]
+    er = np.broadcast_to(0.5 + 0.5j + beta * g, inp["I_epsR_array"].shape)
+    ea = np.broadcast_to(0.5 - 0.5j + gamma * g, inp["I_epsA_array"].shape)
+    return dict(inp, I_epsR_array=np.ascontiguousarray(er),
+                I_epsA_array=np.ascontiguousarray(ea))
+
+
+LINEAR_EPS = (0.3 - 0.2j, -0.15 + 0.05j)
 
 
 SHAPES = [(4, 2, 64, 8, 6, 5), (14, 13, 40, 5, 9, 3), (3, 0, 30, 3, 4, 2),
@@ -112,6 +132,22 @@ def test_oracle_closed_form(shape, sem):
     assert np.all(got["achDtemp"] == 0) and np.all(got["achDtemp_corb"] == 0)
 
 
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("sem", ["cpp", "f90"])
+def test_oracle_linear_in_frequency_closed_form(shape, sem):
+    """I_eps linear in the grid frequency: pins which two slices each
+    interpolation reads and with which weights (a swapped pair, a weight on
+    the wrong slice or the wrong sign of wx changes the value), the slice of
+    achsDtemp, and I_epsA below zero frequency."""
+    inp = linear_eps(gppk.ff_fill_inputs(1, *shape, semantics=sem), *LINEAR_EPS)
+    got = orc.ff_sigma(inp, sem)
+    achs, asx, cor, a2 = closed_form(sem, *shape, *LINEAR_EPS)
+    assert abs(got["achsDtemp"] - achs) <= 1e-12 * abs(achs)
+    assert rel(got["asxDtemp"], asx) <= 1e-12
+    assert rel(got["achDtemp_cor"], cor) <= 1e-12
+    assert rel(got["ach2Dtemp"], a2) <= 1e-12
+
+
 @pytest.mark.parametrize("r", CONST, ids=lambda r: "{number_bands}-{nvband}-{ncouls}-{ngpown}-{nFreq}-{nfreqeval}".format(**r))
 def test_oracle_matches_reference_binary_constfill(r):
     """The lines the unmodified ffKernel.cpp prints (its main hands asxDtemp
@@ -126,6 +162,102 @@ def test_oracle_matches_reference_binary_constfill(r):
     assert abs(achs - complex(*r["achsDtemp"])) <= 1e-6
     assert abs(asx[0] - complex(*r["asxDtemp"])) <= 1e-6
     assert abs(cor[0] - complex(*r["achDtemp_cor"])) <= 1e-6
+
+
+# ---- ffKernel.f90 frequency-grid terms (:335-411) on inputs whose sums
+# have closed forms.  aqsm = 0 removes every I_eps term, so the outputs are
+#   achDtemp      = sum_n1 sum_{f < nF-1} -i pref[f] S[n1][f] / (wx - g[f] + brd[f])
+#   achDtemp_corb = the same with the pole at -g[f] - brd[f] for occupied n1
+#   ach2Dtemp     = sum_n1 sum_{f >= 1} -i pref_zb avg_f it_f,
+#                   it_f = -log|(wx - g[f]) / (wx - g[f-1])|
+#   achDtemp_cor  = sum_n1 sum_{f >= 1} -i pref_zb ((S[f-1] + lin_f/dg_f a) it'_f + lin_f)
+# and the grid, band energies and evaluation frequencies keep every ratio
+# away from the 1e-4 / 1e4 clamps, so log sums telescope.
+GRID = np.array([0.0, 1.0, 2.5, 4.0, 6.0])
+GRID_BRD = np.array([0.3 + 0.2j, -0.1 + 0.4j, 0.2 + 0.1j, 0.05 + 0.3j, 0.1 + 0.25j])
+GRID_PREF = np.array([0.7, 1.1, 0.9, 1.3, 0.6])
+GRID_EKQ = np.array([-12.0, -11.0, -3.2])   # band 0 occupied; band 2's wx
+                                             # runs through the grid
+GRID_C = np.array([0.8 - 0.3j, -0.5 + 1.1j, 0.25 + 0.6j])
+
+
+def grid_problem(schdt):
+    nb, nv, nc, ng, nf, ne = 3, 1, 16, 2, 5, 6
+    inp = gppk.ff_fill_inputs(2, nb, nv, nc, ng, nf, ne, semantics="f90", seed=11)
+    return dict(inp, aqsmtemp=np.zeros_like(inp["aqsmtemp"]), dFreqGrid=GRID.copy(),
+                dFreqBrd=GRID_BRD.copy(), pref=GRID_PREF.copy(), ekq=GRID_EKQ.copy(),
+                schDt_matrix=np.ascontiguousarray(schdt, np.complex128),
+                freqevalmin=0.3, freqevalstep=0.7, pref_zb=0.45)
+
+
+def grid_wx(inp):
+    iw = np.arange(inp["nfreqeval"])
+    return [inp["freqevalmin"] - e + iw * inp["freqevalstep"] for e in inp["ekq"]]
+
+
+def grid_cases():
+    """(name, inputs, {output: expected}) for single-point, constant and
+    linear S(f)."""
+    g, L, pz = GRID, len(GRID) - 1, 0.45
+    cases = []
+    lg = lambda x: np.log(np.abs(x))
+    for k in (0, 2, L):  # S = c delta_{f,k}
+        S = np.zeros((3, len(g)), complex)
+        S[:, k] = GRID_C
+        inp = grid_problem(S)
+        wxs = grid_wx(inp)
+        ach = sum(-1j * GRID_PREF[k] * c / (wx - g[k] + GRID_BRD[k]) for c, wx in zip(GRID_C, wxs))
+        corb = sum(-1j * GRID_PREF[k] * c / (wx - (-g[k] if n1 == 0 else g[k]) + GRID_BRD[k])
+                   for n1, (c, wx) in enumerate(zip(GRID_C, wxs)))
+        if k == L:  # the last grid point has no pole term
+            ach = corb = np.zeros(len(wxs[0]), complex)
+        lo, hi = max(k - 1, 0), min(k + 1, L)
+        a2 = sum(1j * pz * (c / 2) * (lg(wx - g[hi]) - lg(wx - g[lo])) for c, wx in zip(GRID_C, wxs))
+        cases.append((f"delta{k}", inp, {"achDtemp": ach, "achDtemp_corb": corb, "ach2Dtemp": a2}))
+    # S constant per band: lin = 0; the log sums telescope
+    S = np.repeat(GRID_C[:, None], len(g), axis=1)
+    inp = grid_problem(S)
+    wxs = grid_wx(inp)
+    a2 = sum(1j * pz * c * (lg(wx - g[L]) - lg(wx - g[0])) for c, wx in zip(GRID_C, wxs))
+    cor = sum((-1j * pz * c * (lg(wx + g[L]) - lg(wx + g[0])) if n1 == 0 else
+               1j * pz * c * (lg(wx - g[L]) - lg(wx - g[0])))
+              for n1, (c, wx) in enumerate(zip(GRID_C, wxs)))
+    cases.append(("constant", inp, {"ach2Dtemp": a2, "achDtemp_cor": cor}))
+    # S = b g (linear in the grid frequency): lin3_f = b w it'_f + b dg_f with
+    # w = wx (unoccupied; ffKernel.f90:404) or -freqevalmin - ekq + iw step
+    # (occupied; :402), so the sum is b w (sum it') + b (g_L - g_0)
+    S = GRID_C[:, None] * g[None, :]
+    inp = grid_problem(S)
+    wxs = grid_wx(inp)
+    iw = np.arange(inp["nfreqeval"])
+    cor = 0
+    for n1, (b, wx) in enumerate(zip(GRID_C, wxs)):
+        if n1 == 0:
+            w = -inp["freqevalmin"] - inp["ekq"][0] + iw * inp["freqevalstep"]
+            s_it = lg(wx + g[L]) - lg(wx + g[0])
+        else:
+            w = wx
+            s_it = -(lg(wx - g[L]) - lg(wx - g[0]))
+        cor = cor - 1j * pz * (b * w * s_it + b * (g[L] - g[0]))
+    cases.append(("linear", inp, {"achDtemp_cor": cor}))
+    return cases
+
+
+GRID_CASES = grid_cases()
+
+
+@pytest.mark.parametrize("case", GRID_CASES, ids=lambda c: c[0])
+def test_oracle_f90_grid_terms_closed_forms(case):
+    """The oracle's ffKernel.f90 frequency-grid terms against their closed
+    forms: single-point S pins each pole term, wx (iw, not iw-1), the
+    occupied cor pole, the exclusion of the last grid point from the pole
+    sums and the half-sum avg; constant and linear S pin the log ratios, the
+    left-point expansion of lin3 and its + lin term."""
+    name, inp, want = case
+    got = orc.ff_sigma(inp, "f90")
+    assert got["achsDtemp"] == 0 and np.all(got["asxDtemp"] == 0)
+    for k, v in want.items():
+        assert rel(got[k], v) <= 1e-13, (name, k, got[k], v)
 
 
 def test_frequency_search_quirks():

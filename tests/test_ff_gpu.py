@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, gppk, gpp_oracle as orc
-from test_ff_cpu import KATS, SHAPES, c, closed_form, kat_problem, rel
+from test_ff_cpu import (GRID_CASES, KATS, LINEAR_EPS, SHAPES, c, closed_form,
+                         kat_problem, linear_eps, rel)
 
 pytestmark = pytest.mark.gpu
 
@@ -64,6 +65,32 @@ def test_constant_fills_closed_form(handle, shape, sem):
     assert rel(got["asxDtemp"], asx) <= 1e-12
     assert rel(got["achDtemp_cor"], cor) <= 1e-12
     assert rel(got["ach2Dtemp"], a2) <= 1e-12
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("sem", ["cpp", "f90"])
+def test_linear_in_frequency_closed_form(handle, shape, sem):
+    """I_eps linear in the grid frequency: the slice sums and the host's
+    interpolation weights reproduce the linear function at |wx|."""
+    say("ff linear eps", shape, sem)
+    inp = linear_eps(gppk.ff_fill_inputs(1, *shape, semantics=sem), *LINEAR_EPS)
+    got = handle.ff(inp, semantics=sem)
+    achs, asx, cor, a2 = closed_form(sem, *shape, *LINEAR_EPS)
+    assert abs(got["achsDtemp"] - achs) <= 1e-12 * abs(achs)
+    assert rel(got["asxDtemp"], asx) <= 1e-12
+    assert rel(got["achDtemp_cor"], cor) <= 1e-12
+    assert rel(got["ach2Dtemp"], a2) <= 1e-12
+
+
+@pytest.mark.parametrize("case", GRID_CASES, ids=lambda c: c[0])
+def test_f90_grid_terms_closed_forms(handle, case):
+    """The frequency-grid kernel (ffKernel.f90 semantics) against the closed
+    forms of single-point, constant and linear schDt_matrix."""
+    name, inp, want = case
+    say("ff grid closed form", name)
+    got = handle.ff(inp, semantics="f90")
+    for k, v in want.items():
+        assert rel(got[k], v) <= 1e-12, (name, k)
 
 
 def test_device_memory_and_generator(handle):
