@@ -164,6 +164,15 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     p = Path(path) if path else LIB_PATH
     if not p.exists():
         raise OSError(f"GPP library not built: {p} (run __graft_entry__.build())")
+    # PyTorch ships its own HIP runtime (soname libamdhip64.so.7, the same as
+    # /opt/rocm's, which libgpp.so links).  Loaded first, it also serves this
+    # library; loaded second, it would bring a second HIP runtime into the
+    # process, and the two fail to share the device.  So torch, when
+    # present, is imported before the library is opened.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(p))
     vp, ip = C.c_void_p, C.c_int
     lib.gpp_create.argtypes = [C.POINTER(vp), ip]

@@ -8,8 +8,8 @@ import numpy as np
 import pytest
 
 from conftest import GOLDEN, gppk, gpp_oracle as orc
-from test_ff_cpu import (GRID_CASES, KATS, LINEAR_EPS, SHAPES, c, closed_form,
-                         kat_problem, linear_eps, rel)
+from test_ff_cpu import (GRID_CASE_NAMES, KATS, LINEAR_EPS, SHAPES, c,
+                         closed_form, grid_case, kat_problem, linear_eps, rel)
 
 pytestmark = pytest.mark.gpu
 
@@ -82,11 +82,11 @@ def test_linear_in_frequency_closed_form(handle, shape, sem):
     assert rel(got["ach2Dtemp"], a2) <= 1e-12
 
 
-@pytest.mark.parametrize("case", GRID_CASES, ids=lambda c: c[0])
-def test_f90_grid_terms_closed_forms(handle, case):
+@pytest.mark.parametrize("name", GRID_CASE_NAMES)
+def test_f90_grid_terms_closed_forms(handle, name):
     """The frequency-grid kernel (ffKernel.f90 semantics) against the closed
     forms of single-point, constant and linear schDt_matrix."""
-    name, inp, want = case
+    name, inp, want = grid_case(name)
     say("ff grid closed form", name)
     got = handle.ff(inp, semantics="f90")
     for k, v in want.items():
