@@ -180,6 +180,91 @@ int validate(gpp_handle h, const ff_problem *p, const ff_options *o,
   return GPP_OK;
 }
 
+// Frequency-grid terms (achDtemp, achDtemp_corb and the grid parts of
+// ach2Dtemp/achDtemp_cor) on h->ff_stream, after everything queued on `s`
+// so far (the caller's inputs): constants, grid kernel, reduction, copy of
+// the NE x 8 results to h->host_grid; h->ev[3]/ev[4] time the kernels and
+// h->ev[5] marks the results as arrived.  Uses h->ff_grid and h->host_grid
+// only, so the slice sums may run meanwhile.
+int launch_grid_terms(gpp_handle h, const ff_problem *p, bool host, int sem,
+                      hipStream_t s) {
+  int rc = GPP_OK;
+  const int NB = p->number_bands, NV = std::min(p->nvband, NB);
+  const int nF = p->nFreq, NE = p->nfreqeval;
+  if (!h->ff_stream)
+    HIP_OR_RET(h, hipStreamCreateWithFlags(&h->ff_stream, hipStreamNonBlocking));
+  hipStream_t g = h->ff_stream;
+  const int fsplit = gpp::ff::freq_splits(NB, NE, nF);
+  Arena a;
+  const size_t o_part = a.take((size_t)fsplit * NB * NE * gpp::ff::kFreqSlots * 8);
+  const size_t o_fout = a.take((size_t)NE * gpp::ff::kFreqSlots * 8);
+  const size_t o_rows = a.take((size_t)NB * nF * gpp::ff::kRowDoubles * 8);
+  size_t o_ek = 0, o_gr = 0, o_pr = 0, o_br = 0, o_sd = 0;
+  if (host) {
+    o_ek = a.take((size_t)NB * 8);
+    o_gr = a.take((size_t)nF * 8);
+    o_pr = a.take((size_t)nF * 8);
+    o_br = a.take((size_t)nF * 16);
+    o_sd = a.take((size_t)NB * nF * 16);
+  }
+  if ((rc = ensure(h, h->ff_grid, a.used))) return rc;
+  const size_t nres = (size_t)NE * gpp::ff::kFreqSlots;
+  if (h->host_grid_cap < nres) {
+    if (h->host_grid) (void)hipHostFree(h->host_grid);
+    h->host_grid = nullptr;
+    h->host_grid_cap = 0;
+    HIP_OR_RET(h, hipHostMalloc((void **)&h->host_grid, nres * sizeof(double),
+                                hipHostMallocDefault));
+    h->host_grid_cap = nres;
+  }
+  char *base = (char *)h->ff_grid.ptr;
+  gpp::ff::FreqArgs fa{};
+  fa.ekq = p->ekq;
+  fa.grid = p->dFreqGrid;
+  fa.pref = p->pref;
+  fa.brd = (const double2 *)p->dFreqBrd;
+  fa.schdt = (const double2 *)p->schDt_matrix;
+  HIP_OR_RET(h, hipStreamWaitEvent(g, h->ev[0], 0));
+  if (host) {
+    gpp::Stager *st = gpp::stager(h, &rc);
+    if (!st) return rc;
+    const struct {
+      size_t off;
+      const void *src;
+      size_t bytes;
+    } put[] = {{o_ek, p->ekq, (size_t)NB * 8},
+               {o_gr, p->dFreqGrid, (size_t)nF * 8},
+               {o_pr, p->pref, (size_t)nF * 8},
+               {o_br, p->dFreqBrd, (size_t)nF * 16},
+               {o_sd, p->schDt_matrix, (size_t)NB * nF * 16}};
+    for (const auto &c : put) HIP_OR_RET(h, st->copy(base + c.off, c.src, c.bytes));
+    HIP_OR_RET(h, st->fence(g));
+    fa.ekq = (const double *)(base + o_ek);
+    fa.grid = (const double *)(base + o_gr);
+    fa.pref = (const double *)(base + o_pr);
+    fa.brd = (const double2 *)(base + o_br);
+    fa.schdt = (const double2 *)(base + o_sd);
+  }
+  fa.NB = NB;
+  fa.NV = NV;
+  fa.nFreq = nF;
+  fa.NE = NE;
+  fa.fmin = p->freqevalmin;
+  fa.fstep = p->freqevalstep;
+  fa.pref_zb = p->pref_zb;
+  HIP_OR_RET(h, hipEventRecord(h->ev[3], g));
+  HIP_OR_RET(h, gpp::ff::launch_freq(fa, sem, fsplit, (double *)(base + o_rows),
+                                     (double *)(base + o_part), g));
+  HIP_OR_RET(h, gpp::ff::launch_freq_reduce((const double *)(base + o_part),
+                                            fsplit * NB, NE,
+                                            (double *)(base + o_fout), g));
+  HIP_OR_RET(h, hipEventRecord(h->ev[4], g));
+  HIP_OR_RET(h, hipMemcpyAsync(h->host_grid, base + o_fout, nres * 8,
+                               hipMemcpyDeviceToHost, g));
+  HIP_OR_RET(h, hipEventRecord(h->ev[5], g));
+  return GPP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -222,6 +307,21 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   bool sorted = true;
   for (int k = 0; k + 1 < nF && need_grid; ++k)
     sorted = sorted && grid[k] < grid[k + 1];
+
+  // ---- frequency-grid terms: need no plan, so they start first, on a
+  // stream of their own, and run while the host plans and the slice sums
+  // go through the caller's stream.
+  const bool do_ach = (parts & FF_PART_ACH) && NB > 0 && NE > 0;
+  // On every return the grid stream has finished with the handle's buffers.
+  struct GridJoin {
+    gpp_handle h;
+    bool active;
+    ~GridJoin() {
+      if (active && h->ff_stream) (void)hipStreamSynchronize(h->ff_stream);
+    }
+  } join{h, do_ach};
+  HIP_OR_RET(h, hipEventRecord(h->ev[0], s));
+  if (do_ach && (rc = launch_grid_terms(h, p, host, sem, s))) return rc;
 
   // ---- plan: which (kind, slice, band) sums are needed
   // need[(kind*nF + f)*NB + band] -> job*8 + slot, or -1
@@ -321,7 +421,6 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
         std::max(1, NG / 4), std::max<long long>(1, (2048 + per - 1) / per));
   }
   const int per_job = ngroups * nchunk;
-  const bool do_ach = (parts & FF_PART_ACH) && NB > 0 && NE > 0;
   Arena a;
   // coefficient table for up to jbatch passes at a time (at most 1 GiB)
   const int jbatch =
@@ -334,31 +433,15 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   const size_t o_jobs = a.take(jobs.size() * sizeof(SliceJob));
   const size_t o_rec = a.take((size_t)njobs * per_job * gpp::ff::kRecSlots * 8);
   const size_t o_T = a.take((size_t)njobs * gpp::ff::kRecSlots * 8);
-  const int fsplit = do_ach ? gpp::ff::freq_splits(NB, NE, nF) : 1;
-  const size_t o_part = a.take(
-      do_ach ? (size_t)fsplit * NB * NE * gpp::ff::kFreqSlots * 8 : 0);
-  const size_t o_fout = a.take(do_ach ? (size_t)NE * gpp::ff::kFreqSlots * 8 : 0);
-  const size_t o_rows =
-      a.take(do_ach ? (size_t)NB * nF * gpp::ff::kRowDoubles * 8 : 0);
-  // host inputs: small arrays and aqsm/aqsn staged here too
+  // host inputs: aqsm/aqsn and the index arrays staged here too
   const bool need_aqs = njobs > 0;
-  size_t o_aqsm = 0, o_aqsn = 0, o_vc = 0, o_inv = 0, o_ind = 0, o_ek = 0,
-         o_gr = 0, o_pr = 0, o_br = 0, o_sd = 0;
-  if (host) {
-    if (need_aqs) {
-      o_aqsm = a.take((size_t)NB * NC * 16);
-      o_aqsn = a.take((size_t)NB * NC * 16);
-      o_vc = a.take((size_t)NC * 8);
-      o_inv = a.take((size_t)NG * 4);
-      o_ind = a.take((size_t)NC * 4);
-    }
-    if (do_ach) {
-      o_ek = a.take((size_t)NB * 8);
-      o_gr = a.take((size_t)nF * 8);
-      o_pr = a.take((size_t)nF * 8);
-      o_br = a.take((size_t)nF * 16);
-      o_sd = a.take((size_t)NB * nF * 16);
-    }
+  size_t o_aqsm = 0, o_aqsn = 0, o_vc = 0, o_inv = 0, o_ind = 0;
+  if (host && need_aqs) {
+    o_aqsm = a.take((size_t)NB * NC * 16);
+    o_aqsn = a.take((size_t)NB * NC * 16);
+    o_vc = a.take((size_t)NC * 8);
+    o_inv = a.take((size_t)NG * 4);
+    o_ind = a.take((size_t)NC * 4);
   }
   if ((rc = ensure(h, h->ff, a.used))) return rc;
   if (host && !slices.empty() &&
@@ -369,12 +452,6 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   const double2 *aqsn = (const double2 *)p->aqsntemp;
   const double *vcoul = p->vcoul;
   const int *inv = p->inv_igp_index, *indinv = p->indinv;
-  gpp::ff::FreqArgs fa{};
-  fa.ekq = p->ekq;
-  fa.grid = p->dFreqGrid;
-  fa.pref = p->pref;
-  fa.brd = (const double2 *)p->dFreqBrd;
-  fa.schdt = (const double2 *)p->schDt_matrix;
   for (size_t j = 0; j < jobs.size(); ++j) {
     const int kind = slices[job_slice[j]].first, f = slices[job_slice[j]].second;
     if (host)
@@ -404,19 +481,6 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
       inv = (const int *)(base + o_inv);
       indinv = (const int *)(base + o_ind);
     }
-    if (do_ach) {
-      if ((rc = put(o_ek, p->ekq, (size_t)NB * 8)) ||
-          (rc = put(o_gr, p->dFreqGrid, (size_t)nF * 8)) ||
-          (rc = put(o_pr, p->pref, (size_t)nF * 8)) ||
-          (rc = put(o_br, p->dFreqBrd, (size_t)nF * 16)) ||
-          (rc = put(o_sd, p->schDt_matrix, (size_t)NB * nF * 16)))
-        return rc;
-      fa.ekq = (const double *)(base + o_ek);
-      fa.grid = (const double *)(base + o_gr);
-      fa.pref = (const double *)(base + o_pr);
-      fa.brd = (const double2 *)(base + o_br);
-      fa.schdt = (const double2 *)(base + o_sd);
-    }
     for (size_t i = 0; i < slices.size(); ++i) {
       const gpp_c128 *src =
           (slices[i].first == kR ? p->I_epsR_array : p->I_epsA_array) +
@@ -430,12 +494,11 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
     HIP_OR_RET(h, hipMemcpyAsync(base + o_jobs, jobs.data(),
                                  jobs.size() * sizeof(SliceJob),
                                  hipMemcpyHostToDevice, s));
-  const size_t nout = (size_t)njobs * gpp::ff::kRecSlots +
-                      (do_ach ? (size_t)NE * gpp::ff::kFreqSlots : 0);
-  if ((rc = ensure_host_out(h, nout + 1))) return rc;
+  if ((rc = ensure_host_out(h, (size_t)njobs * gpp::ff::kRecSlots + 1)))
+    return rc;
 
-  // ---- kernels
-  HIP_OR_RET(h, hipEventRecord(h->ev[0], s));
+  // ---- slice sums
+  HIP_OR_RET(h, hipEventRecord(h->ev[1], s));
   if (njobs > 0) {
     double2 *cG = (double2 *)(base + o_cg);
     const SliceJob *djobs = (const SliceJob *)(base + o_jobs);
@@ -453,32 +516,11 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
                                                njobs, per_job,
                                                (double *)(base + o_T), s));
   }
-  HIP_OR_RET(h, hipEventRecord(h->ev[1], s));
-  if (do_ach) {
-    fa.NB = NB;
-    fa.NV = NV;
-    fa.nFreq = nF;
-    fa.NE = NE;
-    fa.fmin = p->freqevalmin;
-    fa.fstep = p->freqevalstep;
-    fa.pref_zb = p->pref_zb;
-    HIP_OR_RET(h, gpp::ff::launch_freq(fa, sem, fsplit,
-                                       (double *)(base + o_rows),
-                                       (double *)(base + o_part), s));
-    HIP_OR_RET(h, gpp::ff::launch_freq_reduce((const double *)(base + o_part),
-                                              fsplit * NB, NE,
-                                              (double *)(base + o_fout), s));
-  }
   HIP_OR_RET(h, hipEventRecord(h->ev[2], s));
   double *ho = h->host_out;
   if (njobs > 0)
     HIP_OR_RET(h, hipMemcpyAsync(ho, base + o_T,
                                  (size_t)njobs * gpp::ff::kRecSlots * 8,
-                                 hipMemcpyDeviceToHost, s));
-  double *hf = ho + (size_t)njobs * gpp::ff::kRecSlots;
-  if (do_ach)
-    HIP_OR_RET(h, hipMemcpyAsync(hf, base + o_fout,
-                                 (size_t)NE * gpp::ff::kFreqSlots * 8,
                                  hipMemcpyDeviceToHost, s));
   HIP_OR_RET(h, hipStreamSynchronize(s));
 
@@ -531,6 +573,16 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
           cor[iw].add(-0.5 * v.re, -0.5 * v.im);
         }
       }
+  // the frequency-grid results (their stream may still be running)
+  double asm_ms =
+      std::chrono::duration<double, std::milli>(clk::now() - t_asm).count();
+  if (do_ach) {
+    HIP_OR_RET(h, hipEventSynchronize(h->ev[5]));
+    HIP_OR_RET(h, hipStreamWaitEvent(s, h->ev[5], 0));
+    join.active = false;
+  }
+  const auto t_asm2 = clk::now();
+  const double *hf = h->host_grid;
   for (int iw = 0; iw < NE; ++iw) {
     const double *q = do_ach ? hf + (size_t)iw * gpp::ff::kFreqSlots : nullptr;
     if (r->asxDtemp) r->asxDtemp[iw] = gpp_c128{asx[iw].re, asx[iw].im};
@@ -544,16 +596,21 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
       r->achDtemp_cor[iw] = gpp_c128{cor[iw].re + (q ? q[6] : 0.0),
                                      cor[iw].im + (q ? q[7] : 0.0)};
   }
+  asm_ms += std::chrono::duration<double, std::milli>(clk::now() - t_asm2).count();
   r->host_plan_ms = plan_ms;
-  r->host_assemble_ms =
-      std::chrono::duration<double, std::milli>(clk::now() - t_asm).count();
+  r->host_assemble_ms = asm_ms;
   float ms = 0.f;
-  HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
-  r->slice_kernel_ms = ms;
   HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[1], h->ev[2]));
-  r->freq_kernel_ms = ms;
+  r->slice_kernel_ms = ms;
   HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[2]));
   r->total_ms = ms;
+  r->freq_kernel_ms = 0.0;
+  if (do_ach) {
+    HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[3], h->ev[4]));
+    r->freq_kernel_ms = ms;
+    HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[5]));
+    r->total_ms = std::max(r->total_ms, (double)ms);
+  }
   long long nslice_bands = 0;
   for (const SliceJob &j : jobs) nslice_bands += j.nb;
   r->slices = (long long)slices.size();

@@ -427,15 +427,18 @@ int gpp_destroy(gpp_handle h) {
   if (!h) return GPP_OK;
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  if (h->ff_stream) (void)hipStreamSynchronize(h->ff_stream);
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
-  DevBuf *bufs[] = {&h->stage, &h->work, &h->part, &h->n1log,
-                    &h->meta,  &h->ff,   &h->ff_stage};
+  DevBuf *bufs[] = {&h->stage, &h->work,     &h->part,   &h->n1log,
+                    &h->meta,  &h->ff,       &h->ff_stage, &h->ff_grid};
   for (DevBuf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   for (auto &ev : h->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (h->host_out) (void)hipHostFree(h->host_out);
+  if (h->host_grid) (void)hipHostFree(h->host_grid);
   delete h->stager;
+  if (h->ff_stream) (void)hipStreamDestroy(h->ff_stream);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return GPP_OK;

@@ -73,8 +73,13 @@ struct gpp_handle_s {
   gpp::DevBuf meta;             // batch: problem table, items, record ranges
   gpp::DevBuf ff;               // FF: tables, job list, records, results
   gpp::DevBuf ff_stage;         // FF: staged I_eps slices of host inputs
+  gpp::DevBuf ff_grid;          // FF: frequency-grid constants and sums
+  hipStream_t ff_stream = nullptr;  // FF: frequency-grid terms (created on
+                                    // first use, default priority)
   double *host_out = nullptr;   // pinned result slots
   size_t host_out_cap = 0;      // doubles
+  double *host_grid = nullptr;  // FF: pinned frequency-grid results
+  size_t host_grid_cap = 0;     // doubles
   std::map<gpp::TuneKey, gpp::TileCfg> tuned;
   std::string tune_file;
   // Cached graph of the last batch (replayed when the call is identical).
