@@ -249,16 +249,20 @@ __global__ void freq_prep_kernel(FreqArgs a, double *__restrict__ rows) {
 // A band's rows are wave-uniform (scalar loads).  blockIdx.z picks a
 // contiguous range of f (the grid is split so that small NE*NB still fills
 // the chip); part[((z*NB + n1)*NE + iw)*8].
-template <int SEM>
+// OCC: the launch covers occupied bands only (n1 = blockIdx.y) or
+// unoccupied ones only (n1 = NV + blockIdx.y), so the band kind is a
+// compile-time constant and the unoccupied loop carries one pole division
+// and one logarithm per grid point, not two.
+template <int SEM, bool OCC>
 __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
                                                    const double *__restrict__ rows,
                                                    double *__restrict__ part) {
-  const int n1 = blockIdx.y;
+  const int n1 = OCC ? (int)blockIdx.y : a.NV + (int)blockIdx.y;
   const int iw = blockIdx.x * blockDim.x + threadIdx.x;
   if (iw >= a.NE) return;
   const int f0 = blockIdx.z * fchunk;
   const int f1 = min(a.nFreq, f0 + fchunk);
-  const bool occ = n1 < a.NV;
+  constexpr bool occ = OCC;
   const double ek = a.ekq[n1];
   const double wx = SEM == 0 ? a.fmin - ek + (iw - 1) * a.fstep
                              : a.fmin - ek + iw * a.fstep;
@@ -276,8 +280,7 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
       achi -= t1.y;
       if (SEM == 1) {
         // unoccupied bands: the cor pole is the coh pole (row 3 == row 2)
-        const double2 t2 =
-            occ ? cdiv(num, make_double2(wx - r[3], r[4])) : t1;
+        const double2 t2 = OCC ? cdiv(num, make_double2(wx - r[3], r[4])) : t1;
         cbr -= t2.x;
         cbi -= t2.y;
       }
@@ -393,11 +396,22 @@ hipError_t launch_freq(const FreqArgs &a, int sem, int splits, double *rows,
   if (blocks > 65536) blocks = 65536;
   freq_prep_kernel<<<(unsigned)blocks, 256, 0, s>>>(a, rows);
   const int fchunk = (a.nFreq + splits - 1) / splits;
-  const dim3 grid((a.NE + 255) / 256, a.NB, splits);
-  if (sem == 0)
-    freq_kernel<0><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
-  else
-    freq_kernel<1><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+  const int nocc = std::min(a.NV, a.NB), nun = a.NB - nocc;
+  const unsigned gx = (unsigned)((a.NE + 255) / 256);
+  if (nocc > 0) {
+    const dim3 grid(gx, nocc, splits);
+    if (sem == 0)
+      freq_kernel<0, true><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+    else
+      freq_kernel<1, true><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+  }
+  if (nun > 0) {
+    const dim3 grid(gx, nun, splits);
+    if (sem == 0)
+      freq_kernel<0, false><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+    else
+      freq_kernel<1, false><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+  }
   return hipGetLastError();
 }
 
