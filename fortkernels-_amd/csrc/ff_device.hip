@@ -57,6 +57,12 @@ __device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
   return make_double2((a.x * b.x + a.y * b.y) * r, (a.y * b.x - a.x * b.y) * r);
 }
 
+// The same with b.y * b.y given (by2, rounded as above).
+__device__ __forceinline__ double2 cdiv_y2(double2 a, double2 b, double by2) {
+  const double r = recip(fma(b.x, b.x, by2));
+  return make_double2((a.x * b.x + a.y * b.y) * r, (a.y * b.x - a.x * b.y) * r);
+}
+
 // Coefficient table of a run of passes: for pass j (jobs[j]), row G' and
 // slot b < 8,
 //   cGj[(j*NG + G')*8 + b] = vcoul[igp] conj(aqsm[band_b][igp])
@@ -192,7 +198,7 @@ __device__ __forceinline__ double clamp_ratio(double x) {
 // once (one thread each) with the operations of the reference's loop body
 // (ffKernel.f90:335-384, ffKernel.cpp:200-206); row layout:
 //   0,1  i pref[f] schDt[n1][f]        2  zb - Re brd        3  cor shift
-//   4    Im brd      5  right = zb     6  left = zb[f-1]     7  -
+//   4    Im brd      5  right = zb     6  left = zb[f-1]     7  (Im brd)^2
 //   8,9  i pref_zb avg (sch2 factor)   10,11 schDt[f-1]
 //   12,13 lin/(right-left)             14,15 lin = schDt[f] - schDt[f-1]
 __global__ void freq_prep_kernel(FreqArgs a, double *__restrict__ rows) {
@@ -211,7 +217,7 @@ __global__ void freq_prep_kernel(FreqArgs a, double *__restrict__ rows) {
     r[3] = (occ ? -zb : zb) - br.x;
     r[4] = br.y;
     r[5] = zb;
-    r[7] = 0.0;
+    r[7] = br.y * br.y;
     if (f > 0) {
       const double2 sp = a.schdt[i - 1];
       const double left = a.grid[f - 1];
@@ -275,14 +281,17 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
     const double *r = rb + (size_t)f * kRowDoubles;
     const double2 num = make_double2(r[0], r[1]);
     if (SEM == 0 || f != a.nFreq - 1) {
-      const double2 t1 = cdiv(num, make_double2(wx - r[2], r[4]));
+      const double2 t1 = cdiv_y2(num, make_double2(wx - r[2], r[4]), r[7]);
       achr -= t1.x;
       achi -= t1.y;
       if (SEM == 1) {
-        // unoccupied bands: the cor pole is the coh pole (row 3 == row 2)
-        const double2 t2 = OCC ? cdiv(num, make_double2(wx - r[3], r[4])) : t1;
-        cbr -= t2.x;
-        cbi -= t2.y;
+        // unoccupied bands: the cor pole is the coh pole (row 3 == row 2),
+        // so corb equals ach and is copied after the loop
+        if (OCC) {
+          const double2 t2 = cdiv_y2(num, make_double2(wx - r[3], r[4]), r[7]);
+          cbr -= t2.x;
+          cbi -= t2.y;
+        }
       }
     }
     if (SEM == 1 && f != 0) {
@@ -304,6 +313,10 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
       cr -= -a.pref_zb * l3i;
       ci -= a.pref_zb * l3r;
     }
+  }
+  if (SEM == 1 && !OCC) {
+    cbr = achr;
+    cbi = achi;
   }
   double *o =
       part + (((size_t)blockIdx.z * a.NB + n1) * a.NE + iw) * kFreqSlots;

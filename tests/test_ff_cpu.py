@@ -362,7 +362,7 @@ int main() {
 
 def test_grid_kernel_logarithm_accuracy(tmp_path):
     """The logarithm of the frequency-grid kernel (csrc/ff_math.h, built for
-    the host with g++) against long double logl: at most 1.25 ulp over the
+    the host with g++) against long double logl: at most 1.3 ulp over the
     clamped range [1e-4, 1e4], at most 0.6 ulp within 0.1 of 1."""
     import shutil
     if shutil.which("g++") is None:
@@ -374,5 +374,5 @@ def test_grid_kernel_logarithm_accuracy(tmp_path):
                     str(gppk.PKG_DIR / "csrc"), str(src), "-o", str(exe)], check=True)
     w, n, one, nan = subprocess.run([str(exe)], capture_output=True, text=True,
                                     check=True, timeout=120).stdout.split()
-    assert float(w) <= 1.25 and float(n) <= 0.6, (w, n)
+    assert float(w) <= 1.3 and float(n) <= 0.6, (w, n)
     assert float(one) == 0.0 and nan == "1"
