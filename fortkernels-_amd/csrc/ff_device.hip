@@ -113,32 +113,47 @@ __global__ __launch_bounds__(kSliceThreads) void slice_kernel(
   const int c = blockIdx.x % nchunk, grp = blockIdx.x / nchunk;
   const int rows = (NG + ngroups - 1) / ngroups;
   const int r0 = grp * rows, r1 = min(NG, r0 + rows);
-  const int ig = c * kSliceThreads + threadIdx.x;
-  const bool live = ig < NC;
-  const int igc = live ? ig : NC - 1;
-  double wr[kMaxJobBands], wi[kMaxJobBands];
+  // kSliceG G per thread, kSliceThreads apart (coalesced loads)
+  int ig[kSliceG], igc[kSliceG];
+  bool live[kSliceG];
 #pragma unroll
-  for (int b = 0; b < kMaxJobBands; ++b) wr[b] = wi[b] = 0.0;
-#pragma unroll 4
+  for (int u = 0; u < kSliceG; ++u) {
+    ig[u] = c * kSliceThreads * kSliceG + u * kSliceThreads + threadIdx.x;
+    live[u] = ig[u] < NC;
+    igc[u] = live[u] ? ig[u] : NC - 1;
+  }
+  double wr[kSliceG][kMaxJobBands], wi[kSliceG][kMaxJobBands];
+#pragma unroll
+  for (int u = 0; u < kSliceG; ++u)
+#pragma unroll
+    for (int b = 0; b < kMaxJobBands; ++b) wr[u][b] = wi[u][b] = 0.0;
+#pragma unroll 1
   for (int row = r0; row < r1; ++row) {
-    const double2 y = ldg(sl + (size_t)row * NC + igc);
+    double2 y[kSliceG];
+#pragma unroll
+    for (int u = 0; u < kSliceG; ++u) y[u] = ldg(sl + (size_t)row * NC + igc[u]);
     const double2 *cr = cg + (size_t)row * kMaxJobBands;
 #pragma unroll
     for (int b = 0; b < kMaxJobBands; ++b) {
       const double2 k = cr[b];
-      wr[b] = fma(k.x, y.x, fma(-k.y, y.y, wr[b]));
-      wi[b] = fma(k.x, y.y, fma(k.y, y.x, wi[b]));
+#pragma unroll
+      for (int u = 0; u < kSliceG; ++u) {
+        wr[u][b] = fma(k.x, y[u].x, fma(-k.y, y[u].y, wr[u][b]));
+        wi[u][b] = fma(k.x, y[u].y, fma(k.y, y[u].x, wi[u][b]));
+      }
     }
   }
   double v[kRecSlots];
 #pragma unroll
   for (int b = 0; b < kMaxJobBands; ++b) {
     double pr = 0.0, pi = 0.0;
-    if (b < nb && live) {
-      const double2 a = aqsn[(size_t)job.band[b] * NC + ig];
-      pr = a.x * wr[b] - a.y * wi[b];
-      pi = a.x * wi[b] + a.y * wr[b];
-    }
+#pragma unroll
+    for (int u = 0; u < kSliceG; ++u)
+      if (b < nb && live[u]) {
+        const double2 a = aqsn[(size_t)job.band[b] * NC + ig[u]];
+        pr += a.x * wr[u][b] - a.y * wi[u][b];
+        pi += a.x * wi[u][b] + a.y * wr[u][b];
+      }
     v[2 * b] = pr;
     v[2 * b + 1] = pi;
   }
@@ -350,7 +365,9 @@ __global__ __launch_bounds__(256) void freq_reduce_kernel(
 
 }  // namespace
 
-int slice_chunks(int NC) { return (NC + kSliceThreads - 1) / kSliceThreads; }
+int slice_chunks(int NC) {
+  return (NC + kSliceThreads * kSliceG - 1) / (kSliceThreads * kSliceG);
+}
 
 hipError_t launch_cg(const SliceJob *jobs, int njobs, const double2 *aqsm,
                      const double *vcoul, const int *inv_igp_index,

@@ -12,6 +12,7 @@ namespace ff {
 
 constexpr int kMaxJobBands = 8;   // bands sharing one pass over a slice
 constexpr int kSliceThreads = 256;
+constexpr int kSliceG = 2;        // G columns per slice-kernel thread
 constexpr int kRecSlots = 2 * kMaxJobBands;  // complex per band per record
 constexpr size_t kJobCgBytes = 16 * kMaxJobBands;  // per pass and G' row
 constexpr int kFreqSlots = 8;     // ach, corb, sch2 (grid part), cor (grid
