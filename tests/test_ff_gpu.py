@@ -41,6 +41,31 @@ def test_random_vs_oracle(handle, shape, sem):
     check(got, orc.ff_sigma(inp, sem))
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1, 1, 2, 1), (2, 0, 5, 3, 2, 1),
+                                   (3, 3, 17, 4, 3, 2), (9, 1, 33, 33, 2, 3)])
+@pytest.mark.parametrize("sem", ["cpp", "f90"])
+def test_edge_shapes_vs_oracle(handle, shape, sem):
+    """Single band / column / row, two grid points, one evaluation
+    frequency, nvband = 0 and nvband = number_bands, NG = NC."""
+    say("ff edge", shape, sem)
+    inp = gppk.ff_fill_inputs(2, *shape, semantics=sem, seed=3 + sum(shape))
+    check(handle.ff(inp, semantics=sem), orc.ff_sigma(inp, sem))
+
+
+@pytest.mark.parametrize("sem", ["cpp", "f90"])
+def test_unsorted_grid_vs_oracle(handle, sem):
+    """A frequency grid out of order: the plan falls back from the monotone
+    walker to the reference's linear scans."""
+    say("ff unsorted grid", sem)
+    inp = gppk.ff_fill_inputs(2, 6, 2, 48, 5, 9, 11, semantics=sem, seed=21)
+    perm = np.random.default_rng(5).permutation(9)
+    inp = dict(inp, dFreqGrid=np.ascontiguousarray(inp["dFreqGrid"][perm]),
+               pref=np.ascontiguousarray(inp["pref"][perm]),
+               dFreqBrd=np.ascontiguousarray(inp["dFreqBrd"][perm]))
+    assert np.any(np.diff(inp["dFreqGrid"]) < 0)
+    check(handle.ff(inp, semantics=sem), orc.ff_sigma(inp, sem))
+
+
 @pytest.mark.parametrize("r", KATS, ids=lambda r: f"seed{r['seed']}")
 @pytest.mark.parametrize("sem", ["cpp", "f90"])
 def test_kat_inputs_vs_reference_and_oracle(handle, r, sem):
