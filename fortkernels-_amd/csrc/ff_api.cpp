@@ -2,15 +2,17 @@
 // self-energy terms of FFKer_cori on the GPU.
 //
 // Per call:
-//   1. plan on the host: for every (band, evaluation frequency) the grid
+//   1. device, on a stream of the handle's own: the frequency-grid terms
+//      (they need no plan, so they start first and run through 2-4);
+//   2. plan on the host: for every (band, evaluation frequency) the grid
 //      search of the chosen variant (compute_fact, ffKernel.cpp:28-78, or
 //      ffKernel.f90:231-245,391-430) gives two frequency slices and their
 //      weights; the distinct (array, slice, band) triples are collected
 //      and packed into passes of up to 8 bands per slice;
-//   2. device: cG = vcoul * conj(aqsm) per (band, G'); one HBM pass per
-//      needed slice computes the per-slice sums T(band, slice); the
-//      frequency-grid terms run in their own kernel;
-//   3. host: the outputs are the weighted combinations of the T values
+//   3. device, on the caller's stream: per pass, the coefficients
+//      vcoul * conj(aqsm) of its bands, then one HBM pass over the slice
+//      computes the per-slice sums T(band, slice);
+//   4. host: the outputs are the weighted combinations of the T values
 //      (the interpolations are linear) plus the grid terms.
 // Host inputs are copied through the handle's pinned ring, and only the
 // I_eps slices the evaluation frequencies touch are copied.
@@ -181,13 +183,12 @@ int validate(gpp_handle h, const ff_problem *p, const ff_options *o,
 }
 
 // Frequency-grid terms (achDtemp, achDtemp_corb and the grid parts of
-// ach2Dtemp/achDtemp_cor) on h->ff_stream, after everything queued on `s`
-// so far (the caller's inputs): constants, grid kernel, reduction, copy of
-// the NE x 8 results to h->host_grid; h->ev[3]/ev[4] time the kernels and
-// h->ev[5] marks the results as arrived.  Uses h->ff_grid and h->host_grid
-// only, so the slice sums may run meanwhile.
-int launch_grid_terms(gpp_handle h, const ff_problem *p, bool host, int sem,
-                      hipStream_t s) {
+// ach2Dtemp/achDtemp_cor) on h->ff_stream, after h->ev[0] (recorded by the
+// caller on its stream, behind the inputs): constants, grid kernel,
+// reduction, copy of the NE x 8 results to h->host_grid; h->ev[3]/ev[4]
+// time the kernels and h->ev[5] marks the results as arrived.  Uses
+// h->ff_grid and h->host_grid only, so the slice sums may run meanwhile.
+int launch_grid_terms(gpp_handle h, const ff_problem *p, bool host, int sem) {
   int rc = GPP_OK;
   const int NB = p->number_bands, NV = std::min(p->nvband, NB);
   const int nF = p->nFreq, NE = p->nfreqeval;
@@ -321,7 +322,7 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
     }
   } join{h, do_ach};
   HIP_OR_RET(h, hipEventRecord(h->ev[0], s));
-  if (do_ach && (rc = launch_grid_terms(h, p, host, sem, s))) return rc;
+  if (do_ach && (rc = launch_grid_terms(h, p, host, sem))) return rc;
 
   // ---- plan: which (kind, slice, band) sums are needed
   // need[(kind*nF + f)*NB + band] -> job*8 + slot, or -1
