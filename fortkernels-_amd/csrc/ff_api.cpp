@@ -621,6 +621,8 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   return GPP_OK;
 }
 
+}  // extern "C"
+
 // ---- drop-ins for FFKer_cori/ffKernel.cpp (FF_SEM_CPP, host memory)
 
 namespace {
@@ -644,6 +646,8 @@ void add_into(gpp_c128 *dst, const std::vector<gpp_c128> &v) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int ff_achsDtemp_Kernel(int number_bands, int ngpown, int ncouls,
                         const int *inv_igp_index, const int *indinv,

@@ -95,10 +95,10 @@ typedef struct {
   double host_assemble_ms;        /* host: interpolations of the sums      */
 } ff_result;
 
-void ff_default_options(ff_options *o);
+GPP_API void ff_default_options(ff_options *o);
 
 /* The general entry point. */
-int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
+GPP_API int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
                 ff_result *r);
 
 /*
@@ -110,13 +110,13 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
  * scratch (numThreads, ssxDittt, schDi, schDi_cor, schDi_corb, sch2Di) are
  * accepted and not used.
  */
-int ff_achsDtemp_Kernel(int number_bands, int ngpown, int ncouls,
+GPP_API int ff_achsDtemp_Kernel(int number_bands, int ngpown, int ncouls,
                         const int *inv_igp_index, const int *indinv,
                         const gpp_c128 *aqsntemp, const gpp_c128 *aqsmtemp,
                         const gpp_c128 *I_epsR_array, const double *vcoul,
                         gpp_c128 *achsDtemp);                      /* :107 */
 
-int ff_asxDtemp_Kernel(int nvband, int nfreqeval, int ncouls, int ngpown,
+GPP_API int ff_asxDtemp_Kernel(int nvband, int nfreqeval, int ncouls, int ngpown,
                        int numThreads, int nFreq, double freqevalmin,
                        double freqevalstep, double occ, const double *ekq,
                        const double *dFreqGrid, const int *inv_igp_index,
@@ -125,7 +125,7 @@ int ff_asxDtemp_Kernel(int nvband, int nfreqeval, int ncouls, int ngpown,
                        const gpp_c128 *I_epsR_array, gpp_c128 *ssxDittt,
                        gpp_c128 *asxDtemp);                        /* :133 */
 
-int ff_achDtemp_Kernel(int number_bands, int nvband, int nfreqeval,
+GPP_API int ff_achDtemp_Kernel(int number_bands, int nvband, int nfreqeval,
                        int ncouls, int ngpown, int nFreq, double freqevalmin,
                        double freqevalstep, const double *ekq, double pref_zb,
                        const double *pref, const double *dFreqGrid,
@@ -133,7 +133,7 @@ int ff_achDtemp_Kernel(int number_bands, int nvband, int nfreqeval,
                        gpp_c128 *schDi, gpp_c128 *schDi_cor, gpp_c128 *sch2Di,
                        gpp_c128 *achDtemp);                        /* :155 */
 
-int ff_achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval,
+GPP_API int ff_achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval,
                            int ncouls, int ngpown, int nFreq,
                            double freqevalmin, double freqevalstep,
                            const double *ekq, const double *dFreqGrid,
@@ -152,7 +152,7 @@ int ff_achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval,
  * frequency, sorted ekq in [-10, 10), schDt_matrix ~ N(0,1), strided
  * inv_igp_index).  Scalars (freqevalmin, freqevalstep, pref_zb, occ) are
  * written to *scal[4]. */
-int ff_fill_inputs(int config, int semantics, unsigned long long seed,
+GPP_API int ff_fill_inputs(int config, int semantics, unsigned long long seed,
                    int number_bands, int ncouls, int ngpown, int nFreq,
                    gpp_c128 *aqsm, gpp_c128 *aqsn, gpp_c128 *I_epsR,
                    gpp_c128 *I_epsA, double *vcoul, int *inv_igp_index,
@@ -161,7 +161,7 @@ int ff_fill_inputs(int config, int semantics, unsigned long long seed,
 
 /* The same values generated in device memory on `stream` (hipStream_t,
  * NULL = default); the I_eps arrays are the bulk of an FF problem. */
-int ff_fill_inputs_device(int config, int semantics, unsigned long long seed,
+GPP_API int ff_fill_inputs_device(int config, int semantics, unsigned long long seed,
                           int number_bands, int ncouls, int ngpown, int nFreq,
                           gpp_c128 *aqsm, gpp_c128 *aqsn, gpp_c128 *I_epsR,
                           gpp_c128 *I_epsA, double *vcoul, int *inv_igp_index,

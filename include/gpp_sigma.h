@@ -29,6 +29,11 @@
 #ifndef GPP_SIGMA_H
 #define GPP_SIGMA_H
 
+/* Exported entry points (the library hides every other symbol). */
+#ifndef GPP_API
+#define GPP_API __attribute__((visibility("default")))
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -136,13 +141,13 @@ typedef struct {
 typedef struct gpp_handle_s *gpp_handle;
 
 /* Handle lifetime.  device < 0 selects the current HIP device. */
-int gpp_create(gpp_handle *h, int device);
-int gpp_destroy(gpp_handle h);
-const char *gpp_last_error(gpp_handle h);   /* never NULL */
-void gpp_default_options(gpp_options *o);
+GPP_API int gpp_create(gpp_handle *h, int device);
+GPP_API int gpp_destroy(gpp_handle h);
+GPP_API const char *gpp_last_error(gpp_handle h);   /* never NULL */
+GPP_API void gpp_default_options(gpp_options *o);
 
 /* The general entry point. */
-int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
+GPP_API int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
                  gpp_result *r);
 
 /* Many independent evaluations of uneven size (FP64 or mixed) in one
@@ -151,7 +156,7 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
  * hipGraph that is replayed while the batch (pointers, shapes, options) is
  * unchanged.  All problems use the same options and memory space;
  * band-range flags and acht_n1_loc are not available here. */
-int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
+GPP_API int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
                     const gpp_options *o, gpp_result *r);
 
 /* Tile autotuner: times the non-occupied kernel on a slice of at most 32
@@ -160,7 +165,7 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
  * (precision, semantics, log2 ncouls, log2 ngpown) in the handle, appends it
  * to the file named by $GPP_TUNE_CACHE (read back by gpp_create), and
  * returns it.  Later calls with tile_ig = waves_per_block = 0 use it. */
-int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *o,
+GPP_API int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *o,
                  int *tile_ig, int *waves_per_block);
 
 /*
@@ -170,7 +175,7 @@ int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *o,
  * computes them; |wx| < 1e-6 is clamped to 1e-6 as there.
  * achtemp and asxtemp receive 3 values each.
  */
-int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
+GPP_API int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
               const gpp_c128 *aqsmtemp, const gpp_c128 *aqsntemp,
               const gpp_c128 *I_eps_array, const gpp_c128 *wtilde_array,
               const double *vcoul, const int *inv_igp_index,
@@ -185,7 +190,7 @@ int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
  * n1 < nvband, asxtemp[iw] += ssxt * occ * vcoul[igp].
  * Both take host pointers and run FP64 on the current device.
  */
-int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
+GPP_API int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
                        const gpp_c128 *wtilde_array, const gpp_c128 *aqsmtemp,
                        const gpp_c128 *aqsntemp, const gpp_c128 *I_eps_array,
                        const double *vcoul, const int *inv_igp_index,
@@ -194,7 +199,7 @@ int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
                        double *achtemp_re2, double *achtemp_im0,
                        double *achtemp_im1, double *achtemp_im2);
 
-int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
+GPP_API int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
                           gpp_c128 *asxtemp, const double *wx_array,
                           const gpp_c128 *wtilde_array,
                           const gpp_c128 *aqsmtemp, const gpp_c128 *aqsntemp,
@@ -207,7 +212,7 @@ int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
  * 3 = band-resolved energies with near-resonant wtilde (needs wx[NB][3]).
  * nodes_per_group gives the inv_igp_index stride of configs 2 and 3.
  * Deterministic for (config, seed, shape) on every host. */
-int gpp_fill_inputs(int config, unsigned long long seed, int number_bands,
+GPP_API int gpp_fill_inputs(int config, unsigned long long seed, int number_bands,
                     int ncouls, int ngpown, int nodes_per_group,
                     gpp_c128 *aqsmtemp, gpp_c128 *aqsntemp,
                     gpp_c128 *I_eps_array, gpp_c128 *wtilde_array,
@@ -220,7 +225,7 @@ int gpp_fill_inputs(int config, unsigned long long seed, int number_bands,
  * wx are bit-identical to gpp_fill_inputs; aqsm/aqsn (normal variates) use
  * the device's log/sin/cos and agree with the host's to a few ulp.  For
  * shapes too large to stage from the host on every run. */
-int gpp_fill_inputs_device(int config, unsigned long long seed,
+GPP_API int gpp_fill_inputs_device(int config, unsigned long long seed,
                            int number_bands, int ncouls, int ngpown,
                            int nodes_per_group, gpp_c128 *aqsmtemp,
                            gpp_c128 *aqsntemp, gpp_c128 *I_eps_array,
@@ -229,7 +234,7 @@ int gpp_fill_inputs_device(int config, unsigned long long seed,
                            void *stream);
 
 /* Version string of the library build. */
-const char *gpp_version(void);
+GPP_API const char *gpp_version(void);
 
 #ifdef __cplusplus
 }

@@ -56,11 +56,20 @@ def test_library_exports_every_declared_symbol():
     for name in ("gpp_sigma.h", "ff_sigma.h"):
         header = (ROOT / "include" / name).read_text()
         declared |= set(re.findall(
-            r"^\s*(?:int|void|const char \*)\s*((?:gpp|ff)_\w+)\(", header, re.M))
+            r"^\s*(?:GPP_API\s+)?(?:int|void|const char \*)\s*((?:gpp|ff)_\w+)\(",
+            header, re.M))
     assert declared == set(gppk.EXPORTS), declared ^ set(gppk.EXPORTS)
     for name in declared:
         assert hasattr(lib, name), name
     assert b"gfx950" in lib.gpp_version()
+    # and nothing else: internal C and C++ symbols stay hidden
+    import shutil, subprocess
+    if shutil.which("nm"):
+        out = subprocess.run(["nm", "-D", "--defined-only", str(gppk.LIB_PATH)],
+                             capture_output=True, text=True, check=True).stdout
+        exported = {l.split()[-1] for l in out.splitlines() if l.strip()}
+        exported = {e for e in exported if not e.startswith("__hip_cuid_")}
+        assert exported == declared, exported ^ declared
 
 
 def test_fill_inputs_is_deterministic_and_shaped():
