@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 from conftest import gppk, gpp_oracle as orc, rel_err
-from test_oracle_cpu import closed_form
+from test_oracle_cpu import (achstemp_expected, achstemp_problem, closed_form,
+                             cutoff_expected, cutoff_problem)
 
 pytestmark = pytest.mark.gpu
 
@@ -362,3 +363,38 @@ def test_batch_device_memory(handle):
     for d, r in zip(insts, res):
         host = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in d.items()}
         check(r, orc.sigma(host))
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_occupied_cutoff_closed_form(handle, sem):
+    """The bgw cutoff (|ssx| > 4 |I_eps| and wx < 0) on band-resolved
+    negative and positive frequencies; gppker has none."""
+    say("cutoff closed form", sem)
+    inp = cutoff_problem()
+    got = handle.sigma(inp, semantics=sem)["asxtemp"]
+    want = cutoff_expected(sem, inp)
+    if sem == "bgw":
+        assert got[0] == 0
+    assert rel_err(got, want) <= 1e-12, (got, want)
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_achstemp_single_entries(handle, sem):
+    """The static remainder from three single I_eps entries (on and off the
+    igp column, one below 1e-6)."""
+    say("achstemp single entries", sem)
+    inp, igp = achstemp_problem()
+    got = handle.sigma(inp, semantics=sem)["achstemp"]
+    want = achstemp_expected(sem, inp, igp)
+    assert abs(got - want) <= 1e-12 * abs(want), (got, want)
+
+
+def test_acht_n1_loc_is_each_bands_achtemp2(handle):
+    """Each acht_n1_loc entry against the achtemp[2] of its one-band
+    sub-problem, both through the library."""
+    say("acht_n1_loc per band")
+    inp = gppk.fill_inputs(3, 5, 48, 4, nvband=2, seed=11)
+    full = handle.sigma(inp, acht_n1_loc=True)["acht_n1_loc"]
+    for n1 in range(5):
+        one = handle.sigma(gppk.band_slice(inp, n1, n1 + 1))["achtemp"][2]
+        assert abs(full[n1] - one) <= 1e-12 * abs(one), (n1, full[n1], one)

@@ -311,3 +311,102 @@ def test_bgw_mask_drops_resonant_terms():
     w[0, 5] = 4.0 + 0.0j
     r = orc.sigma(dict(inp, wtilde_array=w), "bgw")
     assert np.all(np.isfinite(r["achtemp"]))
+
+
+# --------------------------------------------------------------------------
+# Closed forms for what the constant fills cannot reach: the bgw cutoff
+# (f90:321, only for wx < 0) and the static remainder achstemp.
+
+CUT_WX = np.array([[-3.48, 3.48, -5.0]])
+
+
+def cutoff_problem():
+    """One occupied band, constant fills except wtilde = 4 (real) and
+    band-resolved wx = (-3.48, 3.48, -5): every term takes the solver's
+    first branch (|wdiff|^2 = (wx - 4)^2 > 0.25, |delw|^2 < 2.5e5) with
+    Omega2 = 16 I_eps, so per term
+      gppker: ssx = Omega2 / wx^2 (its cutoff tests abs(wx) < 0: never),
+      bgw:    ssx = Omega2 conj(cden) / |cden|^2 = 16 c / (wx^2 - 16),
+              zeroed when wx < 0 and |ssx| > 4 |I_eps|:
+    |ssx| = 4.11 |c| at wx = -3.48 (cut), the same at +3.48 (kept: wx > 0),
+    1.78 |c| at -5 (kept)."""
+    inp = gppk.fill_inputs(1, 1, 64, 8, nvband=1)
+    return dict(inp, wtilde_array=np.full_like(inp["wtilde_array"], 4.0),
+                wx_array=CUT_WX.copy())
+
+
+def cutoff_expected(sem, inp):
+    ng, nc = inp["I_eps_array"].shape
+    wx = CUT_WX[0]
+    k = ng * nc * abs(C) ** 2  # sum of matngmatmgp, occ = vcoul = 1
+    if sem == "bgw":
+        ssx = 16 * C / (wx * wx - 16)
+        ssx = np.where((wx < 0) & (np.abs(ssx) > 4 * abs(C)), 0.0, ssx)
+        return -k * ssx
+    return k * 16 * C / wx ** 2
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_oracle_occupied_cutoff_closed_form(sem):
+    inp = cutoff_problem()
+    got = orc.sigma(inp, sem)["asxtemp"]
+    want = cutoff_expected(sem, inp)
+    if sem == "bgw":
+        assert got[0] == 0 and want[0] == 0
+    assert rel_err(got, want) <= 1e-13, (got, want)
+
+
+def achstemp_problem():
+    """Random aqsm, aqsn, vcoul; I_eps zero except three entries:
+    (0, igp(0)) = e1 on the igp column, (2, 5) = e2 off it (5 != igp(2)),
+    and (3, igp(3)) = 1e-8 (below gppKer.cpp's |schs| > 1e-6 test)."""
+    inp = small_random(nb=5, nv=2, nc=48, npg=4, seed=8)
+    ieps = np.zeros_like(inp["I_eps_array"])
+    igp = [int(inp["indinv"][inp["inv_igp_index"][m]]) for m in range(4)]
+    assert igp[2] != 5
+    ieps[0, igp[0]] = 0.7 - 0.2j
+    ieps[2, 5] = -0.3 + 0.9j
+    ieps[3, igp[3]] = 1e-8
+    return dict(inp, I_eps_array=ieps), igp
+
+
+def achstemp_expected(sem, inp, igp):
+    """gppKer.cpp:93-97: sum over bands and rows of aqsn[n1][igp]
+    conj(aqsm[n1][igp]) I_eps[my_igp][igp] vcoul[igp] / 2 when
+    |I_eps[my_igp][igp]| > 1e-6; f90:214-224: minus the sum over ig of
+    aqsn[n1][ig] I_eps[my_igp][ig] conj(aqsm[n1][igp]) vcoul[igp] / 2."""
+    am, an, ie, vc = (inp["aqsmtemp"], inp["aqsntemp"], inp["I_eps_array"],
+                      inp["vcoul"])
+    total = 0j
+    for m, ig in ((0, igp[0]), (2, 5), (3, igp[3])):
+        g = igp[m]
+        e = ie[m, ig]
+        for n1 in range(am.shape[0]):
+            if sem == "bgw":
+                total -= an[n1, ig] * e * np.conj(am[n1, g]) * vc[g] * 0.5
+            elif ig == g and abs(e) > 1e-6:
+                total += an[n1, g] * np.conj(am[n1, g]) * e * vc[g] * 0.5
+    return total
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_oracle_achstemp_single_entries(sem):
+    """Pins which I_eps entries enter the static remainder (the igp column
+    only, above 1e-6, in gppKer.cpp; every ig in the F90), the aqsn/aqsm
+    indices and the signs."""
+    inp, igp = achstemp_problem()
+    got = orc.sigma(inp, sem)["achstemp"]
+    want = achstemp_expected(sem, inp, igp)
+    assert abs(got - want) <= 1e-14 * abs(want), (got, want)
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_oracle_acht_n1_loc_is_each_bands_achtemp2(sem):
+    """acht_n1_loc[n1] (gppKer.cpp:357) is band n1's share of achtemp[2]:
+    the achtemp[2] of the one-band sub-problem (band additivity), for
+    occupied and unoccupied bands alike."""
+    inp = small_random(nb=5, nv=2, nc=48, npg=4, seed=11, config=3)
+    full = orc.sigma(inp, sem, want_acht_n1=True)["acht_n1_loc"]
+    for n1 in range(5):
+        one = orc.sigma(gppk.band_slice(inp, n1, n1 + 1), sem)["achtemp"][2]
+        assert abs(full[n1] - one) <= 1e-13 * abs(one), (n1, full[n1], one)
