@@ -166,9 +166,12 @@ def to_host(d: dict) -> dict:
     return out
 
 
-def check_slice(gppk, h, inp_host: dict, ng_slice: int | None, nb_slice: int):
-    """Oracle check on bands [0, nb_slice) and, if given, the first
-    ng_slice G' columns.  Returns {precision: max rel err}."""
+def cpu_leg_check(gppk, h, inp_host: dict, ng_slice: int | None, nb_slice: int):
+    """CPU leg, checker part: the CPU oracle (the reference's sum restated
+    in C) on bands [0, nb_slice) and, if given, the first ng_slice G'
+    columns, against the library's result for the same sub-problem.  The
+    oracle is only the checker here; nothing timed runs through it.
+    Returns {precision: max rel err}."""
     orc = _load("gpp_oracle", ROOT / "oracle" / "oracle.py")
     sub = gppk.band_slice(inp_host, 0, nb_slice)
     if ng_slice is not None:
@@ -256,10 +259,10 @@ def run_single(args, name):
             f"unocc kernel {ku:.3f} ms, occ kernel {ko:.3f} ms, "
             f"FP64-redone terms {res[-1]['fp64_fallback_terms']}")
 
-    # Error of each path against the FP64 oracle on a slice (all occupied
-    # bands and one unoccupied band; for the large instance only its first
-    # 128 G' columns), and of mixed against FP64 on the full problem,
-    # per achtemp/asxtemp entry.
+    # CPU leg: error of each path against the FP64 oracle on a slice (all
+    # occupied bands and one unoccupied band; for the large instance only
+    # its first 128 G' columns); then mixed against FP64 on the full
+    # problem, per achtemp/asxtemp entry (GPU results only).
     errs = {}
     if not args.no_check:
         host = inp if inp is not None else to_host(
@@ -267,7 +270,7 @@ def run_single(args, name):
                  (k == "wx_array" and v.ndim == 2) else
                  v[:128] if k in ("I_eps_array", "wtilde_array", "inv_igp_index") else v)
              for k, v in dev.items()})
-        e = check_slice(gppk, h, host, None if inp is not None else 128, nv + 1)
+        e = cpu_leg_check(gppk, h, host, None if inp is not None else 128, nv + 1)
         for p in runs:
             errs[p] = e[p]
             log(f"{p}: max rel err vs oracle on the slice {e[p]:.3e}")

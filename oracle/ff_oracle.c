@@ -1,7 +1,7 @@
 /* ff_oracle.c — CPU restatement of the FFKer_cori full-frequency kernels.
  *
- * TEST INFRASTRUCTURE ONLY.  Only tests/ and bench.py's checks load it; the
- * product library never does.  Plain loops in the reference's order of
+ * TEST INFRASTRUCTURE ONLY.  Only tests/ load it (the benchmark checks the
+ * FF path against closed forms); the product library never does.  Plain loops in the reference's order of
  * evaluation (per-element interpolation, then the sums), compensated
  * (Neumaier) accumulation over ig and my_igp, no FMA contraction
  * (-ffp-contract=off), OpenMP over my_igp.

@@ -1,6 +1,6 @@
 /* ff_oracle.h — CPU restatement of the full-frequency (FF) self-energy
- * mini-app of FFKer_cori (TEST INFRASTRUCTURE ONLY: used by tests/ and the
- * benchmark's checks, never by the product library).
+ * mini-app of FFKer_cori (TEST INFRASTRUCTURE ONLY: used by tests/, never
+ * by the product library).
  *
  * Two semantics, as the reference's variants disagree:
  *   FF_ORC_CPP : FFKer_cori/ffKernel.cpp as written (:80-300)

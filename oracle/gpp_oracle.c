@@ -2,8 +2,9 @@
  * gpp_oracle.c — CPU FP64 restatement of the gppKer GPP sigma sum.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the checker the GPU library is
- * compared against.  Only tests/, the smoke check and the CPU-baseline leg
- * of the benchmark load it; the product library never links or calls it.
+ * compared against.  Only tests/, the smoke check and the CPU leg of the
+ * benchmark (baseline timing, result checks) load it; the product library
+ * never links or calls it.
  *
  * What it restates (reference files under rgayatri23/FortKernels-):
  *   gpp_oracle_sigma     the band loop of gppKer.cpp:270-361 with the

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
-benchmark's CPU-baseline leg, as the checker.  The product path never
+benchmark's CPU leg (baseline timing and result checks), as the checker.  The product path never
 imports it.
 """
 from __future__ import annotations
