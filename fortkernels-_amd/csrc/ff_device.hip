@@ -63,6 +63,22 @@ __device__ __forceinline__ double2 cdiv_y2(double2 a, double2 b, double by2) {
   return make_double2((a.x * b.x + a.y * b.y) * r, (a.y * b.x - a.x * b.y) * r);
 }
 
+// 1/d1 and 1/d2 from one reciprocal of d1*d2 (v_rcp_f64 + one Newton step);
+// each through recip() when the product is not a positive normal number.
+__device__ __forceinline__ void recip2(double d1, double d2, double &r1,
+                                       double &r2) {
+  const double p = d1 * d2;
+  if (__builtin_expect(!__builtin_amdgcn_class(p, 0x100), 0)) {
+    r1 = recip(d1);
+    r2 = recip(d2);
+    return;
+  }
+  double q = __builtin_amdgcn_rcp(p);
+  q = fma(q, fma(-p, q, 1.0), q);
+  r1 = q * d2;
+  r2 = q * d1;
+}
+
 // Coefficient table of a run of passes: for pass j (jobs[j]), row G' and
 // slot b < 8,
 //   cGj[(j*NG + G')*8 + b] = vcoul[igp] conj(aqsm[band_b][igp])
@@ -292,7 +308,22 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
   double s2r = 0.0, s2i = 0.0, cr = 0.0, ci = 0.0;
   // occupied lin3 argument: -freqevalmin - ekq + (iw-1)*freqevalstep (f90:375)
   const double wocc = -a.fmin - ek + iw * a.fstep;
-  for (int f = f0; f < f1; ++f) {
+  int f = f0;
+  if constexpr (SEM == 0) {
+    // ffKernel.cpp: pole terms only; two grid points share a reciprocal
+    for (; f + 1 < f1; f += 2) {
+      const double *r = rb + (size_t)f * kRowDoubles;
+      const double *rn = r + kRowDoubles;
+      const double xa = wx - r[2], xb = wx - rn[2];
+      double qa, qb;
+      recip2(fma(xa, xa, r[7]), fma(xb, xb, rn[7]), qa, qb);
+      achr -= (r[0] * xa + r[1] * r[4]) * qa;
+      achi -= (r[1] * xa - r[0] * r[4]) * qa;
+      achr -= (rn[0] * xb + rn[1] * rn[4]) * qb;
+      achi -= (rn[1] * xb - rn[0] * rn[4]) * qb;
+    }
+  }
+  for (; f < f1; ++f) {
     const double *r = rb + (size_t)f * kRowDoubles;
     const double2 num = make_double2(r[0], r[1]);
     if (SEM == 0 || f != a.nFreq - 1) {
