@@ -218,3 +218,19 @@ def test_cli_matches_reference_binary_lines():
     for k in ("achsDtemp", "asxDtemp", "achDtemp_cor"):
         want = "( {:f}, {:f}) ".format(*r[k])
         assert lines[k] == want, (k, lines[k], want)
+
+
+@pytest.mark.parametrize("sem", ["cpp", "f90"])
+def test_cli_lines_match_closed_form(sem):
+    """`ff ... --semantics S` (device-generated mini-app fills) prints the
+    closed-form achsDtemp, asxDtemp[0] and achDtemp_cor[0] (the C++ line of
+    asxDtemp includes achDtemp, zero for these fills)."""
+    import subprocess
+    shape = (8, 2, 64, 8, 6, 5)
+    out = subprocess.run([str(gppk.FF_CLI_PATH)] + [str(x) for x in shape] +
+                         ["--semantics", sem], capture_output=True, text=True,
+                         timeout=120, check=True).stdout
+    lines = {l.split(" = ")[0]: l.split(" = ")[1] for l in out.splitlines() if " = (" in l}
+    achs, asx, cor, _ = closed_form(sem, *shape)
+    for k, v in (("achsDtemp", achs), ("asxDtemp", asx[0]), ("achDtemp_cor", cor[0])):
+        assert lines[k] == "( {:f}, {:f}) ".format(v.real, v.imag), (k, lines[k], v)
