@@ -280,7 +280,8 @@ int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
     const size_t nbnc = (size_t)p[i].number_bands * p[i].ncouls;
     const size_t nbng = (size_t)p[i].number_bands * p[i].ngpown;
     woff[(size_t)i * 4 + 0] = wa.take((size_t)p[i].ngpown * sizeof(int));
-    woff[(size_t)i * 4 + 1] = wa.take((size_t)p[i].number_bands * 3 * 8);
+    // wx [NB][3] (clamped), then the occupied solver's factor [NB][3]
+    woff[(size_t)i * 4 + 1] = wa.take((size_t)p[i].number_bands * 6 * 8);
     woff[(size_t)i * 4 + 2] = mixed ? wa.take(nbnc * sizeof(float2)) : 0;
     woff[(size_t)i * 4 + 3] = wa.take(nbng * 16);
   }

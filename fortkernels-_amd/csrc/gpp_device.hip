@@ -176,6 +176,10 @@ __global__ void prepass_kernel(const int *__restrict__ inv_igp_index,
     double w = wx_per_band ? wx_in[i] : wx_in[i % 3];
     if (fabs(w) < kTo1) w = kTo1;
     wx_out[i] = w;
+    // the occupied solver's first-branch factor cden / |cden|^2 with
+    // cden = wxt^2 (gppKer.cpp:38-39), once per (band, frequency)
+    const double c = w * w;
+    wx_out[NB * 3 + i] = c * recip(c * c);
   }
 }
 
@@ -704,8 +708,7 @@ __device__ __forceinline__ OccBand occ_band(const Prob &p, int n1) {
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {
     B.wxt[iw] = p.wx[n1 * 3 + iw];
-    const double c = B.wxt[iw] * B.wxt[iw];
-    B.fssx[iw] = c * recip(c * c);
+    B.fssx[iw] = p.wx[p.NB * 3 + n1 * 3 + iw];  // formed by prepass_kernel
   }
   return B;
 }

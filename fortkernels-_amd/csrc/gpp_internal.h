@@ -31,7 +31,8 @@ struct DevProblem {
   const double2 *wtilde;  // [NG][NC]
   const double *vcoul;    // [NC]
   const int *igp_of;      // [NG]: resolved igp in [0, NC) or -1 (skipped)
-  const double *wx;       // [NB][3], clamped, band-resolved
+  const double *wx;       // [NB][3], clamped, band-resolved; then [NB][3]
+                          // wx^2 / |wx^2|^2 (occupied solver, gppker)
   const float2 *aqsn32;   // [NB][NC] FP32 copy (mixed precision only)
   const double2 *aqsmG;   // [NB][NG]: aqsm[n1][igp_of[my_igp]], 0 if skipped
   int NB, NV, NC, NG;
