@@ -234,3 +234,30 @@ def test_cli_lines_match_closed_form(sem):
     achs, asx, cor, _ = closed_form(sem, *shape)
     for k, v in (("achsDtemp", achs), ("asxDtemp", asx[0]), ("achDtemp_cor", cor[0])):
         assert lines[k] == "( {:f}, {:f}) ".format(v.real, v.imag), (k, lines[k], v)
+
+
+def _ff_sweep_shapes(n, seed):
+    """Deterministic ragged FF shapes (number_bands, nvband, ncouls, ngpown,
+    nFreq, nfreqeval)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nb = int(rng.integers(1, 20))
+        nv = int(rng.integers(0, nb + 1))
+        nc = int(rng.integers(1, 700))
+        ng = int(rng.integers(1, min(nc, 40) + 1))
+        nf = int(rng.integers(2, 60))
+        ne = int(rng.integers(1, 80))
+        out.append((nb, nv, nc, ng, nf, ne))
+    return out
+
+
+@pytest.mark.parametrize("shape", _ff_sweep_shapes(16, 2025))
+def test_shape_sweep_vs_oracle(handle, shape):
+    """Ragged random shapes in both semantics (several passes per slice when
+    more than 8 bands need it, splits of the frequency sum, odd grid-point
+    counts for the paired reciprocals)."""
+    say("ff sweep", shape)
+    for sem in ("cpp", "f90"):
+        inp = gppk.ff_fill_inputs(2, *shape, semantics=sem, seed=sum(shape) + 1)
+        check(handle.ff(inp, semantics=sem), orc.ff_sigma(inp, sem))

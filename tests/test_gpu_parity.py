@@ -398,3 +398,30 @@ def test_acht_n1_loc_is_each_bands_achtemp2(handle):
     for n1 in range(5):
         one = handle.sigma(gppk.band_slice(inp, n1, n1 + 1))["achtemp"][2]
         assert abs(full[n1] - one) <= 1e-12 * abs(one), (n1, full[n1], one)
+
+
+def _sweep_shapes(n, seed):
+    """Deterministic ragged shapes (number_bands, nvband, ncouls,
+    nodes_per_group) for a sweep: ncouls not a multiple of 64 or of the
+    tile, nvband from 0 to number_bands, one to a few hundred G' rows."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nb = int(rng.integers(1, 12))
+        nv = int(rng.integers(0, nb + 1))
+        npg = int(rng.integers(1, 9))
+        nc = int(rng.integers(max(npg, 2), 1500))
+        out.append((nb, nv, nc, npg))
+    return out
+
+
+@pytest.mark.parametrize("shape", _sweep_shapes(16, 2025))
+@pytest.mark.parametrize("config", [2, 3])
+def test_shape_sweep_vs_oracle(handle, shape, config):
+    """Ragged random shapes, shared (config 2) and band-resolved (config 3)
+    frequencies, every semantics: FP64 within 1e-11 of the oracle."""
+    nb, nv, nc, npg = shape
+    say("sweep", shape, config)
+    inp = gppk.fill_inputs(config, nb, nc, npg, nvband=nv, seed=nb + 7 * nc)
+    for sem in ("gppker", "strict", "bgw"):
+        check(handle.sigma(inp, semantics=sem), orc.sigma(inp, sem))
