@@ -34,6 +34,9 @@ Other workloads (their own JSON line, no CPU baseline):
                  [2,16], nodes_per_group 20) in one batched launch sequence
                  captured in a hipGraph, tile autotuned, against running them
                  one by one with the FP64 path
+  batch_small    the same for 512 small instances (ncouls 256-2048,
+                 number_bands in {16,32,64}, nvband in [2,8],
+                 nodes_per_group 8)
   ff_bw, ff_cb   the full-frequency (FF) kernels at the two example runs of
                  FFKer_cori/ffKernel.f90:9-14 ("bandwidth bound" 8 2 10000
                  600 400 200, "compute bound" 8 2 1000 60 20000 30000),
@@ -487,22 +490,37 @@ def run_ff(args, name):
     return line
 
 
-def run_batch(args):
+BATCHES = {
+    # BASELINE's batch: 64 instances, ncouls log-uniform in [2048, 32768],
+    # number_bands in {128, 256, 512, 1024}, nvband in [2, 16], 20 per group
+    "batch": dict(n=64, nc=(2048, 32768), nb=(128, 256, 512, 1024), nv=(2, 16),
+                  npg=20, desc="64 independent GPP sigma evaluations, batched "
+                               "launches in a hipGraph, autotuned tile"),
+    # many small instances, where per-launch costs and tails matter
+    "batch_small": dict(n=512, nc=(256, 2048), nb=(16, 32, 64), nv=(2, 8), npg=8,
+                        desc="512 small independent GPP sigma evaluations "
+                             "(ncouls 256-2048, 16-64 bands), batched launches "
+                             "in a hipGraph, autotuned tile"),
+}
+
+
+def run_batch(args, name="batch"):
     import torch
     gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
+    w = BATCHES[name]
     rng = np.random.default_rng(args.seed)
     insts = []
-    for i in range(64):
-        nc = int(round(np.exp(rng.uniform(np.log(2048), np.log(32768)))))
-        nb = int(rng.choice([128, 256, 512, 1024]))
-        nv = int(rng.integers(2, 17))
-        insts.append(gppk.fill_inputs_device(3, nb, nc, 20, nvband=nv,
+    for i in range(w["n"]):
+        nc = int(round(np.exp(rng.uniform(np.log(w["nc"][0]), np.log(w["nc"][1])))))
+        nb = int(rng.choice(list(w["nb"])))
+        nv = int(rng.integers(w["nv"][0], w["nv"][1] + 1))
+        insts.append(gppk.fill_inputs_device(3, nb, nc, w["npg"], nvband=nv,
                                              seed=args.seed + 1000 + i))
     torch.cuda.synchronize()
     terms = sum(d["number_bands"] * d["ngpown"] * d["ncouls"] * 3 for d in insts)
     gbytes = sum(16 * (2 * d["number_bands"] + 2 * d["ngpown"]) * d["ncouls"]
                  for d in insts) / 1e9
-    log(f"batch: 64 instances, {terms:.4e} terms, {gbytes:.1f} GB resident")
+    log(f"{name}: {w['n']} instances, {terms:.4e} terms, {gbytes:.1f} GB resident")
     stream = torch.cuda.current_stream()
     h = gppk.Handle(0)
     out = {}
@@ -522,7 +540,7 @@ def run_batch(args):
         dt = (time.perf_counter() - t0) / steps
         out[prec] = {"ms_per_step": dt * 1e3, "terms_per_s": terms / dt,
                      "tile_ig": k, "results": res}
-        log(f"batch {prec}: {dt*1e3:.2f} ms, {terms/dt:.4e} terms/s (tile K={k})")
+        log(f"{name} {prec}: {dt*1e3:.2f} ms, {terms/dt:.4e} terms/s (tile K={k})")
     # one by one, FP64, same handle and stream
     for d in insts:
         h.sigma(d, memory=gppk.MEM_DEVICE, stream=stream.cuda_stream)
@@ -546,9 +564,8 @@ def run_batch(args):
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if best == "fp64" else "mixed(f32 terms, f64 sums)",
         "data": "synthetic",
-        "config": {"workload": "64 independent GPP sigma evaluations, batched "
-                               "launches in a hipGraph, autotuned tile",
-                   "instances": 64, "terms": terms, "resident_GB": gbytes, "path": best},
+        "config": {"workload": w["desc"], "instances": w["n"], "terms": terms,
+                   "resident_GB": gbytes, "path": best},
         "paths": {p: {"terms_per_s": v["terms_per_s"], "ms_per_step": v["ms_per_step"],
                       "tile_ig": v["tile_ig"], "max_rel_err_vs_fp64_one_by_one": errs[p]}
                   for p, v in out.items()},
@@ -566,7 +583,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", default="canonical",
-                    choices=list(WORKLOADS) + ["batch"] + list(FF_WORKLOADS))
+                    choices=list(WORKLOADS) + list(BATCHES) + list(FF_WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -599,8 +616,8 @@ def main():
               "(the sum would shard over G' columns with one all-reduce of 12 "
               "doubles); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.workload == "batch":
-        line = run_batch(args)
+    if args.workload in BATCHES:
+        line = run_batch(args, args.workload)
     elif args.workload in FF_WORKLOADS:
         line = run_ff(args, args.workload)
     else:

@@ -650,6 +650,8 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   if (n < 0 || (n > 0 && (!p || !r)) || !o)
     return set_err(h, GPP_EINVAL, "bad batch arguments");
   if (n == 0) return GPP_OK;
+  if (n > 65535)
+    return set_err(h, GPP_EINVAL, "%d problems in one batch (max 65535)", n);
   int rc;
   for (int i = 0; i < n; ++i) {
     if ((rc = validate(h, p + i, o))) return rc;
@@ -730,11 +732,24 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     nach += ach_count[i];
   }
 
-  // Device metadata: problem table, items, record ranges.
+  // Device metadata: problem table, prepass inputs, items, record ranges.
   std::vector<DevProblem> table(n);
-  for (int i = 0; i < n; ++i) table[i] = prep[i].dp;
+  std::vector<gpp::PrepItem> preps(n);
+  int max_ng_nb3 = 0, max_ach = 0;
+  size_t max_nbng = 0, max_nbnc = 0;
+  for (int i = 0; i < n; ++i) {
+    table[i] = prep[i].dp;
+    preps[i] = gpp::PrepItem{prep[i].inv, prep[i].indinv, prep[i].wx_in,
+                             prep[i].wx_per_band, 0};
+    const DevProblem &d = prep[i].dp;
+    max_ng_nb3 = std::max(max_ng_nb3, std::max(d.NG, 3 * d.NB));
+    max_nbng = std::max(max_nbng, (size_t)d.NB * d.NG);
+    max_nbnc = std::max(max_nbnc, (size_t)d.NB * d.NC);
+    max_ach = std::max(max_ach, ach_count[i]);
+  }
   Arena ma;
   const size_t off_tab = ma.take(sizeof(DevProblem) * n);
+  const size_t off_prep = ma.take(sizeof(gpp::PrepItem) * n);
   const size_t off_items = ma.take(sizeof(BatchItem) * items.size());
   const size_t off_rb = ma.take(sizeof(int) * n);
   const size_t off_rc = ma.take(sizeof(int) * n);
@@ -744,6 +759,8 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   char *mb = (char *)h->meta.ptr;
   std::vector<unsigned char> meta_host(ma.used, 0);
   std::memcpy(meta_host.data() + off_tab, table.data(), sizeof(DevProblem) * n);
+  std::memcpy(meta_host.data() + off_prep, preps.data(),
+              sizeof(gpp::PrepItem) * n);
   std::memcpy(meta_host.data() + off_items, items.data(),
               sizeof(BatchItem) * items.size());
   std::memcpy(meta_host.data() + off_rb, rec_begin.data(), sizeof(int) * n);
@@ -787,8 +804,14 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     hipGraph_t g = nullptr;
     HIP_OR_RET(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int crc = GPP_OK;
-    for (int i = 0; i < n && crc == GPP_OK; ++i)
-      crc = enqueue_prepass(h, prep[i], mixed, dcount, s);
+    {
+      hipError_t e = gpp::launch_prepare_batch(
+          (const DevProblem *)(mb + off_tab),
+          (const gpp::PrepItem *)(mb + off_prep), n, max_ng_nb3, max_nbng,
+          max_nbnc, mixed, dcount, s);
+      if (e != hipSuccess)
+        crc = set_err(h, GPP_EDEVICE, "batch prepass: %s", hipGetErrorString(e));
+    }
     if (crc == GPP_OK) {
       const BatchItem *ditems = (const BatchItem *)(mb + off_items);
       hipError_t e = gpp::launch_batch(
@@ -799,10 +822,10 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
       if (e != hipSuccess)
         crc = set_err(h, GPP_EDEVICE, "batch launch: %s", hipGetErrorString(e));
     }
-    for (int i = 0; i < n && crc == GPP_OK && want_ach; ++i) {
-      hipError_t e = gpp::launch_achstemp(
-          prep[i].dp, o->semantics, part + (size_t)ach_begin[i] * gpp::kSlots,
-          s);
+    if (crc == GPP_OK && want_ach) {
+      hipError_t e = gpp::launch_achstemp_batch(
+          (const DevProblem *)(mb + off_tab), (const int *)(mb + off_ab), n,
+          max_ach, o->semantics, part, s);
       if (e != hipSuccess)
         crc = set_err(h, GPP_EDEVICE, "achstemp: %s", hipGetErrorString(e));
     }

@@ -149,17 +149,12 @@ __device__ __forceinline__ void n1log_store(double re, double im, double *dst,
 // falls outside [0, NC) are skipped (the reference's guard admits igp == NC
 // and then reads one element past the row; here that row is skipped).
 // Clamp of gppKer.cpp:279: |wx| < to1 -> to1.
-__global__ void prepass_kernel(const int *__restrict__ inv_igp_index,
-                               const int *__restrict__ indinv,
-                               const double *__restrict__ wx_in,
-                               int wx_per_band, int NB, int NC, int NG,
-                               int *__restrict__ igp_of,
-                               double *__restrict__ wx_out,
-                               unsigned long long *__restrict__ counter) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  // Cleared by a kernel store, not a memset, so the sequence stays valid
-  // inside a captured graph.
-  if (i == 0) *counter = 0ull;
+__device__ __forceinline__ void prepass_one(int i, const int *__restrict__ inv_igp_index,
+                                            const int *__restrict__ indinv,
+                                            const double *__restrict__ wx_in,
+                                            int wx_per_band, int NB, int NC,
+                                            int NG, int *__restrict__ igp_of,
+                                            double *__restrict__ wx_out) {
   if (i < NG) {
     const int indigp = inv_igp_index[i];
     int igp;
@@ -181,6 +176,21 @@ __global__ void prepass_kernel(const int *__restrict__ inv_igp_index,
     const double c = w * w;
     wx_out[NB * 3 + i] = c * recip(c * c);
   }
+}
+
+__global__ void prepass_kernel(const int *__restrict__ inv_igp_index,
+                               const int *__restrict__ indinv,
+                               const double *__restrict__ wx_in,
+                               int wx_per_band, int NB, int NC, int NG,
+                               int *__restrict__ igp_of,
+                               double *__restrict__ wx_out,
+                               unsigned long long *__restrict__ counter) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // Cleared by a kernel store, not a memset, so the sequence stays valid
+  // inside a captured graph.
+  if (i == 0) *counter = 0ull;
+  prepass_one(i, inv_igp_index, indinv, wx_in, wx_per_band, NB, NC, NG,
+              igp_of, wx_out);
 }
 
 // aqsmG[n1][my_igp] = aqsm[n1][igp_of[my_igp]] (0 for skipped rows): the
@@ -207,6 +217,47 @@ __global__ void to_fp32_kernel(const double2 *__restrict__ in,
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
     const double2 v = in[i];
+    out[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// The same three steps for every problem of a batch in one launch each
+// (problem = blockIdx.y), so a batch of many small problems is not a chain
+// of thousands of tiny launches.
+__global__ void prepass_batch_kernel(const DevProblem *__restrict__ probs,
+                                     const PrepItem *__restrict__ preps,
+                                     unsigned long long *__restrict__ counter) {
+  const DevProblem &p = probs[blockIdx.y];
+  const PrepItem &q = preps[blockIdx.y];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.y == 0 && i == 0) *counter = 0ull;
+  prepass_one(i, q.inv, q.indinv, q.wx_in, q.wx_per_band, p.NB, p.NC, p.NG,
+              (int *)p.igp_of, (double *)p.wx);
+}
+
+__global__ void gather_aqsm_batch_kernel(const DevProblem *__restrict__ probs) {
+  const DevProblem &p = probs[blockIdx.y];
+  const size_t n = (size_t)p.NB * p.NG;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  double2 *out = (double2 *)p.aqsmG;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    const int n1 = (int)(i / p.NG);
+    const int g = (int)(i % p.NG);
+    const int igp = p.igp_of[g];
+    out[i] = igp >= 0 ? p.aqsm[(size_t)n1 * p.NC + igp] : make_double2(0.0, 0.0);
+  }
+}
+
+__global__ void to_fp32_batch_kernel(const DevProblem *__restrict__ probs) {
+  const DevProblem &p = probs[blockIdx.y];
+  if (p.NB <= p.NV || !p.aqsn32) return;
+  const size_t n = (size_t)p.NB * p.NC;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  float2 *out = (float2 *)p.aqsn32;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride) {
+    const double2 v = p.aqsn[i];
     out[i] = make_float2((float)v.x, (float)v.y);
   }
 }
@@ -924,9 +975,9 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(
 // taken when |I_eps[my_igp][igp]| > to1 (:96-97).  One block per my_igp,
 // threads over bands; record slot 0/1.
 constexpr int kAchThreads = 256;
-__global__ __launch_bounds__(kAchThreads) void achstemp_gppker_kernel(
-    DevProblem p, double *__restrict__ partials, int rec_base) {
-  const int my_igp = blockIdx.x;
+__device__ __forceinline__ void achstemp_gppker_block(const DevProblem &p,
+                                                      int my_igp,
+                                                      double *__restrict__ rec) {
   const int igp = p.igp_of[my_igp];
   double vr = 0.0, vi = 0.0;
   if (igp >= 0) {  // block-uniform
@@ -945,9 +996,14 @@ __global__ __launch_bounds__(kAchThreads) void achstemp_gppker_kernel(
     }
   }
   double v[2] = {vr, vi};
-  double *rec = partials + (size_t)(rec_base + blockIdx.x) * kSlots;
   block_store<2>(v, rec);
   if (threadIdx.x >= 2 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
+}
+
+__global__ __launch_bounds__(kAchThreads) void achstemp_gppker_kernel(
+    DevProblem p, double *__restrict__ partials, int rec_base) {
+  achstemp_gppker_block(p, blockIdx.x,
+                        partials + (size_t)(rec_base + blockIdx.x) * kSlots);
 }
 
 // Static remainder, F90 form (gppkernel.deslippe.09.30.2014.f90:195-229):
@@ -955,14 +1011,13 @@ __global__ __launch_bounds__(kAchThreads) void achstemp_gppker_kernel(
 //   I_eps[my_igp][ig]
 // One block per (my_igp, band chunk); lanes over ig.
 constexpr int kAchBands = 8;
-__global__ __launch_bounds__(kAchThreads) void achstemp_bgw_kernel(
-    DevProblem p, double *__restrict__ partials, int rec_base) {
+__device__ __forceinline__ void achstemp_bgw_block(const DevProblem &p, int blk,
+                                                   double *__restrict__ rec) {
   const int nbc = (p.NB + kAchBands - 1) / kAchBands;
-  const int my_igp = blockIdx.x / nbc;
-  const int n1b = (blockIdx.x % nbc) * kAchBands;
+  const int my_igp = blk / nbc;
+  const int n1b = (blk % nbc) * kAchBands;
   const int n1e = min(p.NB, n1b + kAchBands);
   const int igp = p.igp_of[my_igp];
-  double *rec = partials + (size_t)(rec_base + blockIdx.x) * kSlots;
   if (igp < 0) {
     if (threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
     return;
@@ -985,6 +1040,33 @@ __global__ __launch_bounds__(kAchThreads) void achstemp_bgw_kernel(
   double v[2] = {w * vr, w * vi};
   block_store<2>(v, rec);
   if (threadIdx.x >= 2 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
+}
+
+__global__ __launch_bounds__(kAchThreads) void achstemp_bgw_kernel(
+    DevProblem p, double *__restrict__ partials, int rec_base) {
+  achstemp_bgw_block(p, blockIdx.x,
+                     partials + (size_t)(rec_base + blockIdx.x) * kSlots);
+}
+
+__host__ __device__ inline int achstemp_nrec(const DevProblem &p, int sem) {
+  if (p.NG <= 0 || p.NC <= 0 || p.NB <= 0) return 0;
+  if (sem == kSemBgw) return p.NG * ((p.NB + kAchBands - 1) / kAchBands);
+  return p.NG;
+}
+
+// All problems of a batch: problem blockIdx.y, record rec_begin[prob] +
+// blockIdx.x (blocks past a problem's record count return at once).
+template <int SEM>
+__global__ __launch_bounds__(kAchThreads) void achstemp_batch_kernel(
+    const DevProblem *__restrict__ probs, const int *__restrict__ rec_begin,
+    double *__restrict__ partials) {
+  const DevProblem p = probs[blockIdx.y];
+  if ((int)blockIdx.x >= achstemp_nrec(p, SEM)) return;  // block-uniform
+  double *rec = partials + (size_t)(rec_begin[blockIdx.y] + blockIdx.x) * kSlots;
+  if (SEM == kSemBgw)
+    achstemp_bgw_block(p, blockIdx.x, rec);
+  else
+    achstemp_gppker_block(p, blockIdx.x, rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -1251,10 +1333,44 @@ hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,
 }
 
 int achstemp_records(const DevProblem &p, int sem) {
-  if (p.NG <= 0 || p.NC <= 0 || p.NB <= 0) return 0;
+  return achstemp_nrec(p, sem);
+}
+
+hipError_t launch_prepare_batch(const DevProblem *probs, const PrepItem *preps,
+                                int n, int max_ng_nb3, size_t max_nbng,
+                                size_t max_nbnc, bool mixed,
+                                unsigned long long *counter, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (n > 65535) return hipErrorInvalidValue;
+  auto blocks = [](size_t work) {
+    size_t b = (work + 255) / 256;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(b, 4096));
+  };
+  // prepass_one covers one element per thread: no cap on this grid
+  prepass_batch_kernel<<<dim3((unsigned)((max_ng_nb3 + 255) / 256), n), 256, 0,
+                          s>>>(probs, preps, counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  gather_aqsm_batch_kernel<<<dim3(blocks(max_nbng), n), 256, 0, s>>>(probs);
+  e = hipGetLastError();
+  if (e != hipSuccess || !mixed) return e;
+  to_fp32_batch_kernel<<<dim3(blocks(max_nbnc), n), 256, 0, s>>>(probs);
+  return hipGetLastError();
+}
+
+hipError_t launch_achstemp_batch(const DevProblem *probs, const int *rec_begin,
+                                 int n, int max_rec, int sem,
+                                 double *partials, hipStream_t s) {
+  if (n <= 0 || max_rec <= 0) return hipSuccess;
+  if (n > 65535) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)max_rec, n);
   if (sem == kSemBgw)
-    return p.NG * ((p.NB + kAchBands - 1) / kAchBands);
-  return p.NG;
+    achstemp_batch_kernel<kSemBgw><<<grid, kAchThreads, 0, s>>>(probs, rec_begin,
+                                                                 partials);
+  else
+    achstemp_batch_kernel<kSemGppker><<<grid, kAchThreads, 0, s>>>(
+        probs, rec_begin, partials);
+  return hipGetLastError();
 }
 
 hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,

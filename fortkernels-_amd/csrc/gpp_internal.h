@@ -38,6 +38,15 @@ struct DevProblem {
   int NB, NV, NC, NG;
 };
 
+// The per-problem inputs of the prepass that DevProblem does not carry.
+struct PrepItem {
+  const int *inv;         // inv_igp_index [NG]
+  const int *indinv;      // [NC]
+  const double *wx_in;    // [3] or [NB][3]
+  int wx_per_band;
+  int pad;
+};
+
 // Tile shape of the non-occupied kernel.
 struct TileCfg {
   int k_ig = 16;          // G-vectors each lane keeps in registers
@@ -83,6 +92,19 @@ hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,
                             int nb_u, const double *src_o, int nrec_o,
                             int nb_o, int NB, double *out, hipStream_t s);
 int achstemp_records(const DevProblem &p, int sem);
+// Prepass, aqsm gather and (mixed) FP32 copy of aqsn for problems
+// [0, n) of a device table, one launch each (problem = blockIdx.y);
+// max_ng_nb3 = max over problems of max(NG, 3 NB), max_nbng / max_nbnc the
+// largest NB*NG / NB*NC.
+hipError_t launch_prepare_batch(const DevProblem *probs, const PrepItem *preps,
+                                int n, int max_ng_nb3, size_t max_nbng,
+                                size_t max_nbnc, bool mixed,
+                                unsigned long long *counter, hipStream_t s);
+// achstemp records of problems [0, n): problem i writes records
+// rec_begin[i] + [0, achstemp_records(i)); max_rec is the largest count.
+hipError_t launch_achstemp_batch(const DevProblem *probs, const int *rec_begin,
+                                 int n, int max_rec, int sem,
+                                 double *partials, hipStream_t s);
 hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
                            hipStream_t s);
 // One reduction block per problem: records [rec_begin, +rec_count) ->
