@@ -33,6 +33,7 @@ GPP_EUNSUPPORTED = -5
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_FP64_FACTORED = 2
 SEM_GPPKER = 0
 SEM_GPPKER_STRICT = 1
 SEM_BGW = 2
@@ -45,7 +46,8 @@ FLAG_NO_ACHSTEMP = 8
 FLAG_AUTOTUNE = 16
 
 SEMANTICS = {"gppker": SEM_GPPKER, "strict": SEM_GPPKER_STRICT, "bgw": SEM_BGW}
-PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
+PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED,
+             "fp64_factored": PREC_FP64_FACTORED}
 
 FF_SEM_CPP = 0
 FF_SEM_F90 = 1

@@ -132,8 +132,13 @@ int validate(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
                    p->number_bands, p->nvband, p->ncouls, p->ngpown);
   if ((long long)p->number_bands * 3 > 0x7fffffffLL)
     return set_err(h, GPP_EINVAL, "number_bands too large");
-  if (o->precision != GPP_PREC_FP64 && o->precision != GPP_PREC_MIXED)
+  if (o->precision != GPP_PREC_FP64 && o->precision != GPP_PREC_MIXED &&
+      o->precision != GPP_PREC_FP64_FACTORED)
     return set_err(h, GPP_EINVAL, "bad precision %d", o->precision);
+  if (o->precision == GPP_PREC_FP64_FACTORED && p->wx_per_band &&
+      p->number_bands > 0)
+    return set_err(h, GPP_EUNSUPPORTED,
+                   "the factored band sum needs wx shared by all bands");
   if (o->semantics < GPP_SEM_GPPKER || o->semantics > GPP_SEM_BGW)
     return set_err(h, GPP_EINVAL, "bad semantics %d", o->semantics);
   if (o->memory != GPP_MEM_HOST && o->memory != GPP_MEM_DEVICE)
@@ -467,8 +472,13 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   const bool want_ach = !(o->flags & GPP_FLAG_NO_ACHSTEMP);
   const bool want_n1 = r->acht_n1_loc != nullptr;
 
-  if ((o->flags & GPP_FLAG_AUTOTUNE) && !o->tile_ig && !o->waves_per_block &&
-      !h->tuned.count(tune_key(p, o))) {
+  const bool factored = o->precision == GPP_PREC_FP64_FACTORED;
+  if (factored && want_n1)
+    return set_err(h, GPP_EUNSUPPORTED,
+                   "acht_n1_loc needs per-band sums; the factored band sum "
+                   "has none");
+  if ((o->flags & GPP_FLAG_AUTOTUNE) && !factored && !o->tile_ig &&
+      !o->waves_per_block && !h->tuned.count(tune_key(p, o))) {
     int k, w;
     if ((rc = gpp_autotune(h, p, o, &k, &w))) return rc;
   }
@@ -489,7 +499,9 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   std::vector<Prepared> prep;
   if ((rc = prepare(h, p, 1, o, mixed, s, prep, pipelined))) return rc;
   const DevProblem &dp = prep[0].dp;
-  const int n_unocc = run_unocc ? gpp::unocc_grid(dp, tile) : 0;
+  const int n_unocc = !run_unocc ? 0
+                      : factored ? gpp::unocc_gemm_grid(dp)
+                                 : gpp::unocc_grid(dp, tile);
   const int n_occ = run_occ ? gpp::occ_grid(dp) : 0;
   const int n_ach = want_ach ? gpp::achstemp_records(dp, o->semantics) : 0;
   const int nrec = n_unocc + n_occ;
@@ -534,9 +546,13 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
                                p->wtilde_array + off, len));
         HIP_OR_RET(h, st->fence(s));
       }
-      HIP_OR_RET(h, gpp::launch_unocc(dp, n1u_begin, NB, o->precision,
-                                      o->semantics, tile, thr, part, n1u,
-                                      mixed ? dcount : nullptr, s, g, cnt));
+      if (factored)
+        HIP_OR_RET(h, gpp::launch_unocc_gemm(dp, n1u_begin, NB, o->semantics,
+                                             part, s, g, cnt));
+      else
+        HIP_OR_RET(h, gpp::launch_unocc(dp, n1u_begin, NB, o->precision,
+                                        o->semantics, tile, thr, part, n1u,
+                                        mixed ? dcount : nullptr, s, g, cnt));
     }
   }
   HIP_OR_RET(h, hipEventRecord(h->ev[2], s));
@@ -598,6 +614,9 @@ int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *o,
     return set_err(h, GPP_EINVAL, "null output");
   int rc = validate(h, p, o);
   if (rc) return rc;
+  if (o->precision == GPP_PREC_FP64_FACTORED)
+    return set_err(h, GPP_EUNSUPPORTED,
+                   "the factored band sum has no tile to tune");
   // Time the non-occupied kernel on a slice of at most 32 unoccupied bands
   // (cost is linear in bands) for every candidate tile.
   gpp_problem q = *p;
@@ -662,6 +681,9 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   if (o->flags & (GPP_FLAG_ALL_BANDS_UNOCC | GPP_FLAG_SKIP_UNOCC))
     return set_err(h, GPP_EUNSUPPORTED,
                    "band-range flags are not supported in batches");
+  if (o->precision == GPP_PREC_FP64_FACTORED)
+    return set_err(h, GPP_EUNSUPPORTED,
+                   "the factored band sum is not available in batches");
   HIP_OR_RET(h, hipSetDevice(h->device));
   hipStream_t s = (o->memory == GPP_MEM_DEVICE && o->stream)
                       ? (hipStream_t)o->stream

@@ -703,6 +703,216 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Non-occupied bands, band sum first (GPP_PREC_FP64_FACTORED).
+// When wx does not depend on the band, neither does delw, and the sum of
+// gppKer.cpp:316-345 factors:
+//   achtemp[iw] = sum_{my_igp} 0.5 vcoul[igp] sum_ig P[my_igp][ig] *
+//                 wtilde*I_eps * conj(wdiff_iw) / |wdiff_iw|^2
+//   P[my_igp][ig] = sum_{n1} conj(aqsm[n1][igp]) * aqsn[n1][ig]
+// P is a complex matrix product over the bands (K = n1_end - n1_begin),
+// formed on the FP64 matrix cores as four real products:
+//   Re P = Ar^T Br + Ai^T Bi,  Im P = Ar^T Bi - Ai^T Br
+// (A = aqsmG [n1][my_igp], B = aqsn [n1][ig]).  A block owns 64 G' columns
+// (one group, as the other kernels) by 128 G-vectors; each of its four
+// waves a 32 x 64 quarter: 2 x 4 tiles of v_mfma_f64_16x16x4 (16 x 16
+// outputs, 4 bands per step).  Operands come straight from global memory:
+// in one step a lane reads one complex of A per row tile and one of B per
+// column tile (16 lanes cover 256 contiguous bytes of a band's row).  P
+// stays in registers and the epilogue contracts it with delw * I_eps of
+// the same (my_igp, ig) read once from HBM; P is never stored.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int kGemmM = 64;   // G' columns per block
+constexpr int kGemmN = 128;  // G-vectors per block
+constexpr int kGemmThreads = 256;
+
+__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Blocks are handed to the XCDs round-robin (block x runs on XCD x % 8).
+// Renumber them so that each XCD runs a contiguous range of tiles: the
+// tiles of one G-vector range (which share the aqsn slab) then share that
+// XCD's L2.
+__device__ __forceinline__ int xcd_contiguous(int x, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd;
+  return x < q * kXcd ? (x % kXcd) * q + x / kXcd : x;
+}
+
+// One step of 4 bands: a[mb] = A[n1][row tile mb], b[nb] = B[n1][col tile nb]
+// for this lane's (band, row/column) slot.
+__device__ __forceinline__ void gemm_step(const double2 (&a)[2],
+                                          const double2 (&b)[4],
+                                          dbl4 (&pr)[2][4], dbl4 (&pi)[2][4]) {
+  // First all accumulators once, then all again, so no MFMA waits on the
+  // previous one's result.
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      pr[mb][nb] = mfma4(a[mb].x, b[nb].x, pr[mb][nb]);
+      pi[mb][nb] = mfma4(a[mb].x, b[nb].y, pi[mb][nb]);
+    }
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb) {
+    const double nai = -a[mb].y;
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      pr[mb][nb] = mfma4(a[mb].y, b[nb].y, pr[mb][nb]);
+      pi[mb][nb] = mfma4(nai, b[nb].x, pi[mb][nb]);
+    }
+  }
+}
+
+template <bool MASK>
+__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(2))) void unocc_gemm_kernel(
+    DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
+    double *__restrict__ partials) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = uniform(threadIdx.x / kWave);
+  const int NC = p.NC, NG = p.NG;
+  const int ngrp = (NG + kGemmM - 1) / kGemmM;
+  // Tile of this block, numbered as in the launch over all groups (so a
+  // pipelined launch over a range of groups writes the same records).
+  const int x = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int b = (x / grp_cnt) * ngrp + grp_lo + x % grp_cnt;
+  const int igp_grp = b % ngrp, ig_tile = b / ngrp;
+  const int m0 = igp_grp * kGemmM + (wave & 1) * 32;
+  const int c0 = ig_tile * kGemmN + (wave >> 1) * 64;
+  const int li = lane & 15, lk = lane >> 4;
+
+  // Operand columns of this lane.  Rows past NG and columns past NC are
+  // clamped to real data (finite) and get zero weight in the epilogue.
+  const double2 *ap[2];
+  const double2 *bp[4];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+    ap[mb] = p.aqsmG + min(m0 + 16 * mb + li, NG - 1);
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb)
+    bp[nb] = p.aqsn + min(c0 + 16 * nb + li, NC - 1);
+
+  dbl4 pr[2][4], pi[2][4];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      pr[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      pi[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  const int nsteps = (n1_end - n1_begin) / 4;
+  const size_t sa = (size_t)4 * NG, sb = (size_t)4 * NC;
+  size_t oa = (size_t)(n1_begin + lk) * NG, ob = (size_t)(n1_begin + lk) * NC;
+  double2 a[2], bv[4];
+  if (nsteps > 0) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) a[mb] = ap[mb][oa];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) bv[nb] = bp[nb][ob];
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    // Next step's operands (the last iteration reloads its own step: in
+    // bounds, and no branch in the loop).
+    const bool more = s + 1 < nsteps;
+    oa += more ? sa : 0;
+    ob += more ? sb : 0;
+    double2 an[2], bn[4];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) an[mb] = ap[mb][oa];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) bn[nb] = bp[nb][ob];
+    gemm_step(a, bv, pr, pi);
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) a[mb] = an[mb];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) bv[nb] = bn[nb];
+  }
+  // Remaining 1-3 bands: lanes whose band is past n1_end contribute zeros.
+  const int n1t = n1_begin + 4 * nsteps + lk;
+  if (n1_begin + 4 * nsteps < n1_end) {
+    const bool in = n1t < n1_end;
+    const size_t ta = (size_t)min(n1t, n1_end - 1) * NG;
+    const size_t tb = (size_t)min(n1t, n1_end - 1) * NC;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      const double2 v = ap[mb][ta];
+      a[mb] = in ? v : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) {
+      const double2 v = bp[nb][tb];
+      bv[nb] = in ? v : make_double2(0.0, 0.0);
+    }
+    gemm_step(a, bv, pr, pi);
+  }
+
+  // Epilogue.  Output (i, j) of a 16 x 16 tile sits in lane 16*(i%4) + j,
+  // element i/4 (operands: A[i][k] in lane 16k + i, B[k][j] in lane
+  // 16k + j).
+  const double x0 = p.wx[n1_begin * 3 + 0];
+  const double x1 = p.wx[n1_begin * 3 + 1];
+  const double x2 = p.wx[n1_begin * 3 + 2];
+  // The element loop stays rolled (its index picks the accumulator element
+  // by selects): unrolled, the compiler hoists all 64 complex loads of the
+  // epilogue and spills.
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      const int my_igp = m0 + 16 * mb + lk + 4 * e;
+      const int igp = my_igp < NG ? p.igp_of[my_igp] : -1;
+      const bool valid = igp >= 0;
+      const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
+      double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) {
+        const int ig = c0 + 16 * nb + li;
+        double wr = 0.0, wi = 0.0, Ar = 0.0, Ai = 0.0;
+        if (valid && ig < NC) {
+          const size_t off = (size_t)my_igp * NC + ig;
+          const double2 w = p.wtilde[off];
+          const double2 ep = p.ieps[off];
+          wr = w.x;
+          wi = w.y;
+          Ar = w.x * ep.x - w.y * ep.y;
+          Ai = w.x * ep.y + w.y * ep.x;
+        }
+        const double Pr = pr[mb][nb][e], Pi = pi[mb][nb][e];
+        const double Br = Pr * Ar - Pi * Ai;
+        const double Bi = Pr * Ai + Pi * Ar;
+        const double wi2 = wi * wi;
+        const double dr0 = x0 - wr, dr1 = x1 - wr, dr2 = x2 - wr;
+        const double d0 = fma(dr0, dr0, wi2), d1 = fma(dr1, dr1, wi2),
+                     d2 = fma(dr2, dr2, wi2);
+        double r0, r1, r2;
+        recip3(d0, d1, d2, r0, r1, r2);
+        if (MASK) {
+          // gppkernel.deslippe.09.30.2014.f90:364-366: keep the term only if
+          // |wdiff|^2 > limittwo and |wtilde|^2/|wdiff|^2 < limitone.
+          const double wt2 = wr * wr + wi2;
+          r0 = (d0 > kLimitTwo && wt2 * r0 < kLimitOne) ? r0 : 0.0;
+          r1 = (d1 > kLimitTwo && wt2 * r1 < kLimitOne) ? r1 : 0.0;
+          r2 = (d2 > kLimitTwo && wt2 * r2 < kLimitOne) ? r2 : 0.0;
+        }
+        const double nBiwi = -Bi * wi, Brwi = Br * wi;
+        s[0] = fma(fma(Br, dr0, nBiwi), r0, s[0]);
+        s[1] = fma(fma(Bi, dr0, Brwi), r0, s[1]);
+        s[2] = fma(fma(Br, dr1, nBiwi), r1, s[2]);
+        s[3] = fma(fma(Bi, dr1, Brwi), r1, s[3]);
+        s[4] = fma(fma(Br, dr2, nBiwi), r2, s[4]);
+        s[5] = fma(fma(Bi, dr2, Brwi), r2, s[5]);
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc[i] = fma(wgt, s[i], acc[i]);
+    }
+  double *rec = partials + (size_t)b * kSlots;
+  block_store<6>(acc, rec);
+  if (threadIdx.x >= 6 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
 // Occupied bands (n1 < nvband): ssxt_scht_solver / flagOCC_solver of
 // gppKer.cpp:16-62,122-146 (SEM gppker, gppker-strict) or the occupied
 // branch of gppkernel.deslippe.09.30.2014.f90:271-334 (SEM bgw).
@@ -1279,6 +1489,33 @@ hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
     default:
       return hipErrorInvalidValue;
   }
+}
+
+int unocc_gemm_grid(const DevProblem &p) {
+  if (p.NG <= 0 || p.NC <= 0) return 0;
+  return unocc_groups(p) * ((p.NC + kGemmN - 1) / kGemmN);
+}
+
+hipError_t launch_unocc_gemm(const DevProblem &p, int n1_begin, int n1_end,
+                             int sem, double *partials, hipStream_t s,
+                             int grp_lo, int grp_cnt) {
+  static_assert(kGemmM == kWave, "a block covers one group of G' columns");
+  const int ngrp = unocc_groups(p);
+  if (grp_cnt < 0) {
+    grp_lo = 0;
+    grp_cnt = ngrp;
+  }
+  if (grp_lo < 0 || grp_lo + grp_cnt > ngrp) return hipErrorInvalidValue;
+  const int full = unocc_gemm_grid(p);
+  if (full == 0 || grp_cnt == 0 || n1_begin >= n1_end) return hipSuccess;
+  const int grid = full / ngrp * grp_cnt;
+  if (sem == kSemBgw)
+    unocc_gemm_kernel<true><<<grid, kGemmThreads, 0, s>>>(
+        p, n1_begin, n1_end, grp_lo, grp_cnt, partials);
+  else
+    unocc_gemm_kernel<false><<<grid, kGemmThreads, 0, s>>>(
+        p, n1_begin, n1_end, grp_lo, grp_cnt, partials);
+  return hipGetLastError();
 }
 
 int occ_grid(const DevProblem &p) {

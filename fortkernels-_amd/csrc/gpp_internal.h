@@ -82,6 +82,13 @@ hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
                         double res_thresh, double *partials, double *n1log,
                         unsigned long long *fallback_count, hipStream_t s,
                         int grp_lo = 0, int grp_cnt = -1);
+// Band-sum-first variant (wx shared by all bands): one block per 64 G'
+// columns x 128 G-vectors; same record and group conventions as
+// launch_unocc.
+int unocc_gemm_grid(const DevProblem &p);
+hipError_t launch_unocc_gemm(const DevProblem &p, int n1_begin, int n1_end,
+                             int sem, double *partials, hipStream_t s,
+                             int grp_lo = 0, int grp_cnt = -1);
 int occ_grid(const DevProblem &p);
 // n1log: NULL, or [grid][n1_end] complex per-band records.
 hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
