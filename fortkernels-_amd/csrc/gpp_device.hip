@@ -703,216 +703,6 @@ __global__ __launch_bounds__(256) void unocc_mixed_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Non-occupied bands, band sum first (GPP_PREC_FP64_FACTORED).
-// When wx does not depend on the band, neither does delw, and the sum of
-// gppKer.cpp:316-345 factors:
-//   achtemp[iw] = sum_{my_igp} 0.5 vcoul[igp] sum_ig P[my_igp][ig] *
-//                 wtilde*I_eps * conj(wdiff_iw) / |wdiff_iw|^2
-//   P[my_igp][ig] = sum_{n1} conj(aqsm[n1][igp]) * aqsn[n1][ig]
-// P is a complex matrix product over the bands (K = n1_end - n1_begin),
-// formed on the FP64 matrix cores as four real products:
-//   Re P = Ar^T Br + Ai^T Bi,  Im P = Ar^T Bi - Ai^T Br
-// (A = aqsmG [n1][my_igp], B = aqsn [n1][ig]).  A block owns 64 G' columns
-// (one group, as the other kernels) by 128 G-vectors; each of its four
-// waves a 32 x 64 quarter: 2 x 4 tiles of v_mfma_f64_16x16x4 (16 x 16
-// outputs, 4 bands per step).  Operands come straight from global memory:
-// in one step a lane reads one complex of A per row tile and one of B per
-// column tile (16 lanes cover 256 contiguous bytes of a band's row).  P
-// stays in registers and the epilogue contracts it with delw * I_eps of
-// the same (my_igp, ig) read once from HBM; P is never stored.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-constexpr int kGemmM = 64;   // G' columns per block
-constexpr int kGemmN = 128;  // G-vectors per block
-constexpr int kGemmThreads = 256;
-
-__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// Blocks are handed to the XCDs round-robin (block x runs on XCD x % 8).
-// Renumber them so that each XCD runs a contiguous range of tiles: the
-// tiles of one G-vector range (which share the aqsn slab) then share that
-// XCD's L2.
-__device__ __forceinline__ int xcd_contiguous(int x, int n) {
-  constexpr int kXcd = 8;
-  const int q = n / kXcd;
-  return x < q * kXcd ? (x % kXcd) * q + x / kXcd : x;
-}
-
-// One step of 4 bands: a[mb] = A[n1][row tile mb], b[nb] = B[n1][col tile nb]
-// for this lane's (band, row/column) slot.
-__device__ __forceinline__ void gemm_step(const double2 (&a)[2],
-                                          const double2 (&b)[4],
-                                          dbl4 (&pr)[2][4], dbl4 (&pi)[2][4]) {
-  // First all accumulators once, then all again, so no MFMA waits on the
-  // previous one's result.
-#pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-      pr[mb][nb] = mfma4(a[mb].x, b[nb].x, pr[mb][nb]);
-      pi[mb][nb] = mfma4(a[mb].x, b[nb].y, pi[mb][nb]);
-    }
-#pragma unroll
-  for (int mb = 0; mb < 2; ++mb) {
-    const double nai = -a[mb].y;
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-      pr[mb][nb] = mfma4(a[mb].y, b[nb].y, pr[mb][nb]);
-      pi[mb][nb] = mfma4(nai, b[nb].x, pi[mb][nb]);
-    }
-  }
-}
-
-template <bool MASK>
-__global__ __launch_bounds__(kGemmThreads) __attribute__((amdgpu_waves_per_eu(2))) void unocc_gemm_kernel(
-    DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
-    double *__restrict__ partials) {
-  const int lane = threadIdx.x & (kWave - 1);
-  const int wave = uniform(threadIdx.x / kWave);
-  const int NC = p.NC, NG = p.NG;
-  const int ngrp = (NG + kGemmM - 1) / kGemmM;
-  // Tile of this block, numbered as in the launch over all groups (so a
-  // pipelined launch over a range of groups writes the same records).
-  const int x = xcd_contiguous(blockIdx.x, gridDim.x);
-  const int b = (x / grp_cnt) * ngrp + grp_lo + x % grp_cnt;
-  const int igp_grp = b % ngrp, ig_tile = b / ngrp;
-  const int m0 = igp_grp * kGemmM + (wave & 1) * 32;
-  const int c0 = ig_tile * kGemmN + (wave >> 1) * 64;
-  const int li = lane & 15, lk = lane >> 4;
-
-  // Operand columns of this lane.  Rows past NG and columns past NC are
-  // clamped to real data (finite) and get zero weight in the epilogue.
-  const double2 *ap[2];
-  const double2 *bp[4];
-#pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
-    ap[mb] = p.aqsmG + min(m0 + 16 * mb + li, NG - 1);
-#pragma unroll
-  for (int nb = 0; nb < 4; ++nb)
-    bp[nb] = p.aqsn + min(c0 + 16 * nb + li, NC - 1);
-
-  dbl4 pr[2][4], pi[2][4];
-#pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-      pr[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
-      pi[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
-    }
-
-  const int nsteps = (n1_end - n1_begin) / 4;
-  const size_t sa = (size_t)4 * NG, sb = (size_t)4 * NC;
-  size_t oa = (size_t)(n1_begin + lk) * NG, ob = (size_t)(n1_begin + lk) * NC;
-  double2 a[2], bv[4];
-  if (nsteps > 0) {
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) a[mb] = ap[mb][oa];
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) bv[nb] = bp[nb][ob];
-  }
-  for (int s = 0; s < nsteps; ++s) {
-    // Next step's operands (the last iteration reloads its own step: in
-    // bounds, and no branch in the loop).
-    const bool more = s + 1 < nsteps;
-    oa += more ? sa : 0;
-    ob += more ? sb : 0;
-    double2 an[2], bn[4];
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) an[mb] = ap[mb][oa];
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) bn[nb] = bp[nb][ob];
-    gemm_step(a, bv, pr, pi);
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) a[mb] = an[mb];
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) bv[nb] = bn[nb];
-  }
-  // Remaining 1-3 bands: lanes whose band is past n1_end contribute zeros.
-  const int n1t = n1_begin + 4 * nsteps + lk;
-  if (n1_begin + 4 * nsteps < n1_end) {
-    const bool in = n1t < n1_end;
-    const size_t ta = (size_t)min(n1t, n1_end - 1) * NG;
-    const size_t tb = (size_t)min(n1t, n1_end - 1) * NC;
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-      const double2 v = ap[mb][ta];
-      a[mb] = in ? v : make_double2(0.0, 0.0);
-    }
-#pragma unroll
-    for (int nb = 0; nb < 4; ++nb) {
-      const double2 v = bp[nb][tb];
-      bv[nb] = in ? v : make_double2(0.0, 0.0);
-    }
-    gemm_step(a, bv, pr, pi);
-  }
-
-  // Epilogue.  Output (i, j) of a 16 x 16 tile sits in lane 16*(i%4) + j,
-  // element i/4 (operands: A[i][k] in lane 16k + i, B[k][j] in lane
-  // 16k + j).
-  const double x0 = p.wx[n1_begin * 3 + 0];
-  const double x1 = p.wx[n1_begin * 3 + 1];
-  const double x2 = p.wx[n1_begin * 3 + 2];
-  // The element loop stays rolled (its index picks the accumulator element
-  // by selects): unrolled, the compiler hoists all 64 complex loads of the
-  // epilogue and spills.
-  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll 1
-  for (int e = 0; e < 4; ++e)
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-      const int my_igp = m0 + 16 * mb + lk + 4 * e;
-      const int igp = my_igp < NG ? p.igp_of[my_igp] : -1;
-      const bool valid = igp >= 0;
-      const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
-      double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb) {
-        const int ig = c0 + 16 * nb + li;
-        double wr = 0.0, wi = 0.0, Ar = 0.0, Ai = 0.0;
-        if (valid && ig < NC) {
-          const size_t off = (size_t)my_igp * NC + ig;
-          const double2 w = p.wtilde[off];
-          const double2 ep = p.ieps[off];
-          wr = w.x;
-          wi = w.y;
-          Ar = w.x * ep.x - w.y * ep.y;
-          Ai = w.x * ep.y + w.y * ep.x;
-        }
-        const double Pr = pr[mb][nb][e], Pi = pi[mb][nb][e];
-        const double Br = Pr * Ar - Pi * Ai;
-        const double Bi = Pr * Ai + Pi * Ar;
-        const double wi2 = wi * wi;
-        const double dr0 = x0 - wr, dr1 = x1 - wr, dr2 = x2 - wr;
-        const double d0 = fma(dr0, dr0, wi2), d1 = fma(dr1, dr1, wi2),
-                     d2 = fma(dr2, dr2, wi2);
-        double r0, r1, r2;
-        recip3(d0, d1, d2, r0, r1, r2);
-        if (MASK) {
-          // gppkernel.deslippe.09.30.2014.f90:364-366: keep the term only if
-          // |wdiff|^2 > limittwo and |wtilde|^2/|wdiff|^2 < limitone.
-          const double wt2 = wr * wr + wi2;
-          r0 = (d0 > kLimitTwo && wt2 * r0 < kLimitOne) ? r0 : 0.0;
-          r1 = (d1 > kLimitTwo && wt2 * r1 < kLimitOne) ? r1 : 0.0;
-          r2 = (d2 > kLimitTwo && wt2 * r2 < kLimitOne) ? r2 : 0.0;
-        }
-        const double nBiwi = -Bi * wi, Brwi = Br * wi;
-        s[0] = fma(fma(Br, dr0, nBiwi), r0, s[0]);
-        s[1] = fma(fma(Bi, dr0, Brwi), r0, s[1]);
-        s[2] = fma(fma(Br, dr1, nBiwi), r1, s[2]);
-        s[3] = fma(fma(Bi, dr1, Brwi), r1, s[3]);
-        s[4] = fma(fma(Br, dr2, nBiwi), r2, s[4]);
-        s[5] = fma(fma(Bi, dr2, Brwi), r2, s[5]);
-      }
-#pragma unroll
-      for (int i = 0; i < 6; ++i) acc[i] = fma(wgt, s[i], acc[i]);
-    }
-  double *rec = partials + (size_t)b * kSlots;
-  block_store<6>(acc, rec);
-  if (threadIdx.x >= 6 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
-}
-
-// ---------------------------------------------------------------------------
 // Occupied bands (n1 < nvband): ssxt_scht_solver / flagOCC_solver of
 // gppKer.cpp:16-62,122-146 (SEM gppker, gppker-strict) or the occupied
 // branch of gppkernel.deslippe.09.30.2014.f90:271-334 (SEM bgw).
@@ -933,16 +723,8 @@ struct OccG {
   double cut2;         // (sexcut * |I_eps|)^2
 };
 
-template <int SEM, typename Prob>
-__device__ __forceinline__ OccG occ_g(const Prob &p, int my_igp, int ig) {
-  const int NC = p.NC, NG = p.NG;
-  const size_t off = (size_t)my_igp * NC + ig;
-  const double2 w = ld(p.wtilde + off);
-  const double2 e = ld(p.ieps + off);
-  // gppKer.cpp:139 passes I_eps_array[my_igp*ngpown+ig] to the solver; the
-  // strict mode keeps that index, the others read [my_igp][ig].
-  const double2 ex =
-      (SEM == kSemGppkerStrict) ? ld(p.ieps + (size_t)my_igp * NG + ig) : e;
+// From wtilde, I_eps[my_igp][ig] and ex, the I_eps entry the solver reads.
+__device__ __forceinline__ OccG occ_g_of(double2 w, double2 e, double2 ex) {
   OccG g;
   g.wr = w.x;
   g.wi = w.y;
@@ -955,6 +737,19 @@ __device__ __forceinline__ OccG occ_g(const Prob &p, int my_igp, int ig) {
   g.exi = ex.y;
   g.cut2 = kSexcut * kSexcut * (ex.x * ex.x + ex.y * ex.y);
   return g;
+}
+
+template <int SEM, typename Prob>
+__device__ __forceinline__ OccG occ_g(const Prob &p, int my_igp, int ig) {
+  const int NC = p.NC, NG = p.NG;
+  const size_t off = (size_t)my_igp * NC + ig;
+  const double2 w = ld(p.wtilde + off);
+  const double2 e = ld(p.ieps + off);
+  // gppKer.cpp:139 passes I_eps_array[my_igp*ngpown+ig] to the solver; the
+  // strict mode keeps that index, the others read [my_igp][ig].
+  const double2 ex =
+      (SEM == kSemGppkerStrict) ? ld(p.ieps + (size_t)my_igp * NG + ig) : e;
+  return occ_g_of(w, e, ex);
 }
 
 // The band's frequencies and, for the gppker first branch, the factor
@@ -1114,6 +909,224 @@ __global__ __launch_bounds__(kOccThreads) void occ_kernel(
                         partials + (size_t)blockIdx.x * kSlots, n1log);
 }
 
+// ---------------------------------------------------------------------------
+// Non-occupied bands, band sum first (GPP_PREC_FP64_FACTORED).
+// When wx does not depend on the band, neither does delw, and the sum of
+// gppKer.cpp:316-345 factors:
+//   achtemp[iw] = sum_{my_igp} 0.5 vcoul[igp] sum_ig P[my_igp][ig] *
+//                 wtilde*I_eps * conj(wdiff_iw) / |wdiff_iw|^2
+//   P[my_igp][ig] = sum_{n1} conj(aqsm[n1][igp]) * aqsn[n1][ig]
+// P is a complex matrix product over the bands (K = n1_end - n1_begin),
+// formed on the FP64 matrix cores as four real products:
+//   Re P = Ar^T Br + Ai^T Bi,  Im P = Ar^T Bi - Ai^T Br
+// (A = aqsmG [n1][my_igp], B = aqsn [n1][ig]).  A block owns 64 G' columns
+// (one group, as the other kernels) by 64 G-vectors; each of its four waves
+// a 32 x 32 quarter: 2 x 2 tiles of v_mfma_f64_16x16x4 (16 x 16 outputs, 4
+// bands per step).  That leaves room for 3 waves per SIMD, which hide the
+// operand loads better than 2 waves of 32 x 64 quarters (3.8-3.9 ms against
+// 4.5-4.6 ms at 512/2/32768/20 on MI355X).  Operands come straight from
+// global memory:
+// in one step a lane reads one complex of A per row tile and one of B per
+// column tile (16 lanes cover 256 contiguous bytes of a band's row).  P
+// stays in registers and the epilogue contracts it with delw * I_eps of
+// the same (my_igp, ig) read once from HBM; P is never stored.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int kGemmM = 64;   // G' columns per block
+constexpr int kGemmNB = 2;   // 16-wide column tiles per wave
+constexpr int kGemmN = 2 * 16 * kGemmNB;  // G-vectors per block
+constexpr int kGemmThreads = 256;
+constexpr int kGemmWaves = 3;  // waves per SIMD the registers are sized for
+
+__device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Blocks are handed to the XCDs round-robin (block x runs on XCD x % 8).
+// Renumber them so that each XCD runs a contiguous range of tiles: the
+// tiles of one G-vector range (which share the aqsn slab) then share that
+// XCD's L2.
+__device__ __forceinline__ int xcd_contiguous(int x, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd;
+  return x < q * kXcd ? (x % kXcd) * q + x / kXcd : x;
+}
+
+// One step of 4 bands: a[mb] = A[n1][row tile mb], b[nb] = B[n1][col tile nb]
+// for this lane's (band, row/column) slot.
+__device__ __forceinline__ void gemm_step(const double2 (&a)[2],
+                                          const double2 (&b)[kGemmNB],
+                                          dbl4 (&pr)[2][kGemmNB],
+                                          dbl4 (&pi)[2][kGemmNB]) {
+  // First all accumulators once, then all again, so no MFMA waits on the
+  // previous one's result.
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < kGemmNB; ++nb) {
+      pr[mb][nb] = mfma4(a[mb].x, b[nb].x, pr[mb][nb]);
+      pi[mb][nb] = mfma4(a[mb].x, b[nb].y, pi[mb][nb]);
+    }
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb) {
+    const double nai = -a[mb].y;
+#pragma unroll
+    for (int nb = 0; nb < kGemmNB; ++nb) {
+      pr[mb][nb] = mfma4(a[mb].y, b[nb].y, pr[mb][nb]);
+      pi[mb][nb] = mfma4(nai, b[nb].x, pi[mb][nb]);
+    }
+  }
+}
+
+template <bool MASK>
+__global__ __launch_bounds__(kGemmThreads) __attribute__((
+    amdgpu_waves_per_eu(kGemmWaves))) void unocc_gemm_kernel(
+    DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
+    double *__restrict__ partials) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = uniform(threadIdx.x / kWave);
+  const int NC = p.NC, NG = p.NG;
+  const int ngrp = (NG + kGemmM - 1) / kGemmM;
+  // Tile of this block, numbered as in the launch over all groups (so a
+  // pipelined launch over a range of groups writes the same records).
+  const int x = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int b = (x / grp_cnt) * ngrp + grp_lo + x % grp_cnt;
+  const int igp_grp = b % ngrp, ig_tile = b / ngrp;
+  const int m0 = igp_grp * kGemmM + (wave & 1) * 32;
+  const int c0 = ig_tile * kGemmN + (wave >> 1) * 16 * kGemmNB;
+  const int li = lane & 15, lk = lane >> 4;
+
+  // Operand columns of this lane.  Rows past NG and columns past NC are
+  // clamped to real data (finite) and get zero weight in the epilogue.
+  const double2 *ap[2];
+  const double2 *bp[kGemmNB];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+    ap[mb] = p.aqsmG + min(m0 + 16 * mb + li, NG - 1);
+#pragma unroll
+  for (int nb = 0; nb < kGemmNB; ++nb)
+    bp[nb] = p.aqsn + min(c0 + 16 * nb + li, NC - 1);
+
+  dbl4 pr[2][kGemmNB], pi[2][kGemmNB];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < kGemmNB; ++nb) {
+      pr[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      pi[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  const int nsteps = (n1_end - n1_begin) / 4;
+  const size_t sa = (size_t)4 * NG, sb = (size_t)4 * NC;
+  size_t oa = (size_t)(n1_begin + lk) * NG, ob = (size_t)(n1_begin + lk) * NC;
+  double2 a[2], bv[kGemmNB];
+  if (nsteps > 0) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) a[mb] = ap[mb][oa];
+#pragma unroll
+    for (int nb = 0; nb < kGemmNB; ++nb) bv[nb] = bp[nb][ob];
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    // Next step's operands (the last iteration reloads its own step: in
+    // bounds, and no branch in the loop).
+    const bool more = s + 1 < nsteps;
+    oa += more ? sa : 0;
+    ob += more ? sb : 0;
+    double2 an[2], bn[kGemmNB];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) an[mb] = ap[mb][oa];
+#pragma unroll
+    for (int nb = 0; nb < kGemmNB; ++nb) bn[nb] = bp[nb][ob];
+    gemm_step(a, bv, pr, pi);
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) a[mb] = an[mb];
+#pragma unroll
+    for (int nb = 0; nb < kGemmNB; ++nb) bv[nb] = bn[nb];
+  }
+  // Remaining 1-3 bands: lanes whose band is past n1_end contribute zeros.
+  const int n1t = n1_begin + 4 * nsteps + lk;
+  if (n1_begin + 4 * nsteps < n1_end) {
+    const bool in = n1t < n1_end;
+    const size_t ta = (size_t)min(n1t, n1_end - 1) * NG;
+    const size_t tb = (size_t)min(n1t, n1_end - 1) * NC;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      const double2 v = ap[mb][ta];
+      a[mb] = in ? v : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int nb = 0; nb < kGemmNB; ++nb) {
+      const double2 v = bp[nb][tb];
+      bv[nb] = in ? v : make_double2(0.0, 0.0);
+    }
+    gemm_step(a, bv, pr, pi);
+  }
+
+  // Epilogue.  Output (i, j) of a 16 x 16 tile sits in lane 16*(i%4) + j,
+  // element i/4 (operands: A[i][k] in lane 16k + i, B[k][j] in lane
+  // 16k + j).
+  const double x0 = p.wx[n1_begin * 3 + 0];
+  const double x1 = p.wx[n1_begin * 3 + 1];
+  const double x2 = p.wx[n1_begin * 3 + 2];
+  // The element loop stays rolled (its index picks the accumulator element
+  // by selects): unrolled, the compiler hoists all 64 complex loads of the
+  // epilogue and spills.
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int e = 0; e < 4; ++e)
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) {
+      const int my_igp = m0 + 16 * mb + lk + 4 * e;
+      const int igp = my_igp < NG ? p.igp_of[my_igp] : -1;
+      const bool valid = igp >= 0;
+      const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
+      double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int nb = 0; nb < kGemmNB; ++nb) {
+        const int ig = c0 + 16 * nb + li;
+        double wr = 0.0, wi = 0.0, Ar = 0.0, Ai = 0.0;
+        if (valid && ig < NC) {
+          const size_t off = (size_t)my_igp * NC + ig;
+          const double2 w = p.wtilde[off];
+          const double2 ep = p.ieps[off];
+          wr = w.x;
+          wi = w.y;
+          Ar = w.x * ep.x - w.y * ep.y;
+          Ai = w.x * ep.y + w.y * ep.x;
+        }
+        const double Pr = pr[mb][nb][e], Pi = pi[mb][nb][e];
+        const double Br = Pr * Ar - Pi * Ai;
+        const double Bi = Pr * Ai + Pi * Ar;
+        const double wi2 = wi * wi;
+        const double dr0 = x0 - wr, dr1 = x1 - wr, dr2 = x2 - wr;
+        const double d0 = fma(dr0, dr0, wi2), d1 = fma(dr1, dr1, wi2),
+                     d2 = fma(dr2, dr2, wi2);
+        double r0, r1, r2;
+        recip3(d0, d1, d2, r0, r1, r2);
+        if (MASK) {
+          // gppkernel.deslippe.09.30.2014.f90:364-366: keep the term only if
+          // |wdiff|^2 > limittwo and |wtilde|^2/|wdiff|^2 < limitone.
+          const double wt2 = wr * wr + wi2;
+          r0 = (d0 > kLimitTwo && wt2 * r0 < kLimitOne) ? r0 : 0.0;
+          r1 = (d1 > kLimitTwo && wt2 * r1 < kLimitOne) ? r1 : 0.0;
+          r2 = (d2 > kLimitTwo && wt2 * r2 < kLimitOne) ? r2 : 0.0;
+        }
+        const double nBiwi = -Bi * wi, Brwi = Br * wi;
+        s[0] = fma(fma(Br, dr0, nBiwi), r0, s[0]);
+        s[1] = fma(fma(Bi, dr0, Brwi), r0, s[1]);
+        s[2] = fma(fma(Br, dr1, nBiwi), r1, s[2]);
+        s[3] = fma(fma(Bi, dr1, Brwi), r1, s[3]);
+        s[4] = fma(fma(Br, dr2, nBiwi), r2, s[4]);
+        s[5] = fma(fma(Bi, dr2, Brwi), r2, s[5]);
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc[i] = fma(wgt, s[i], acc[i]);
+    }
+  double *rec = partials + (size_t)b * kSlots;
+  block_store<6>(acc, rec);
+  if (threadIdx.x >= 6 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
+
+}
+
 // Per-band log reduction: out[n1] = sum over the unoccupied-kernel blocks
 // (records [rec][n1 - u_begin]) plus sum over the occupied-kernel blocks
 // (records [rec][n1]), each in record order.  One block per band.
@@ -1158,18 +1171,41 @@ __global__ __launch_bounds__(256) void reduce_n1_kernel(
 // One block per slot: thread t sums records t, t+256, ... in order
 // (Neumaier-compensated), then a fixed pairwise tree over the 256 partials.
 constexpr int kRedThreads = 256;
+
+// Neumaier-compensated acc += x.
+__device__ __forceinline__ void neumaier_add(double &acc, double &c, double x) {
+  const double t = acc + x;
+  c += (fabs(acc) >= fabs(x)) ? ((acc - t) + x) : ((x - t) + acc);
+  acc = t;
+}
+
+// This thread's records r = threadIdx.x + k * kRedThreads, k = 0, 1, ...
+// in that order, slot s, compensated.  Eight loads are issued before their
+// eight adds, so a thread waits for memory once per eight records (the
+// order of the adds, and so the result, is that of the plain loop).
+__device__ __forceinline__ double strided_sum(const double *__restrict__ part,
+                                              int nrec, int s) {
+  double acc = 0.0, c = 0.0;
+  int r = threadIdx.x;
+  constexpr int U = 8;
+  for (; r + (U - 1) * kRedThreads < nrec; r += U * kRedThreads) {
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      x[u] = part[(size_t)(r + u * kRedThreads) * kSlots + s];
+#pragma unroll
+    for (int u = 0; u < U; ++u) neumaier_add(acc, c, x[u]);
+  }
+  for (; r < nrec; r += kRedThreads)
+    neumaier_add(acc, c, part[(size_t)r * kSlots + s]);
+  return acc + c;
+}
+
 __global__ __launch_bounds__(kRedThreads) void reduce_kernel(
     const double *__restrict__ partials, int nrec, double *__restrict__ out) {
   __shared__ double buf[kRedThreads];
   const int s = blockIdx.x;
-  double acc = 0.0, c = 0.0;
-  for (int r = threadIdx.x; r < nrec; r += kRedThreads) {
-    const double x = partials[(size_t)r * kSlots + s];
-    const double t = acc + x;
-    c += (fabs(acc) >= fabs(x)) ? ((acc - t) + x) : ((x - t) + acc);
-    acc = t;
-  }
-  buf[threadIdx.x] = acc + c;
+  buf[threadIdx.x] = strided_sum(partials, nrec, s);
   __syncthreads();
   for (int h = kRedThreads / 2; h > 0; h >>= 1) {
     if (threadIdx.x < h) buf[threadIdx.x] += buf[threadIdx.x + h];
@@ -1356,14 +1392,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_batch_kernel(
   const double *part = partials + (size_t)rec_begin[blockIdx.x] * kSlots;
   const int nrec = rec_count[blockIdx.x];
   for (int s = 0; s < kSlots; ++s) {
-    double acc = 0.0, c = 0.0;
-    for (int r = threadIdx.x; r < nrec; r += kRedThreads) {
-      const double x = part[(size_t)r * kSlots + s];
-      const double t = acc + x;
-      c += (fabs(acc) >= fabs(x)) ? ((acc - t) + x) : ((x - t) + acc);
-      acc = t;
-    }
-    buf[threadIdx.x] = acc + c;
+    buf[threadIdx.x] = strided_sum(part, nrec, s);
     __syncthreads();
     for (int h = kRedThreads / 2; h > 0; h >>= 1) {
       if (threadIdx.x < h) buf[threadIdx.x] += buf[threadIdx.x + h];

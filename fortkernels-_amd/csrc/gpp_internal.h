@@ -83,7 +83,7 @@ hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
                         unsigned long long *fallback_count, hipStream_t s,
                         int grp_lo = 0, int grp_cnt = -1);
 // Band-sum-first variant (wx shared by all bands): one block per 64 G'
-// columns x 128 G-vectors; same record and group conventions as
+// columns x 64 G-vectors; same record and group conventions as
 // launch_unocc.
 int unocc_gemm_grid(const DevProblem &p);
 hipError_t launch_unocc_gemm(const DevProblem &p, int n1_begin, int n1_end,

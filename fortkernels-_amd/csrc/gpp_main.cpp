@@ -1,7 +1,7 @@
 // gpp_main.cpp — the gppKer mini-app driver on top of the C ABI.
 //
 //   gpp <number_bands> <number_valence_bands> <number_plane_waves>
-//       <nodes_per_mpi_group> [--precision=fp64|mixed]
+//       <nodes_per_mpi_group> [--precision=fp64|mixed|factored]
 //       [--semantics=gppker|strict|bgw] [--config=1|2|3] [--seed=N]
 //
 // Same positional arguments, usage text, banners and result lines as
@@ -47,8 +47,9 @@ int main(int argc, char **argv) {
       return a.compare(0, n, k) == 0 ? a.c_str() + n : nullptr;
     };
     if (const char *v = val("--precision=")) {
-      opt.precision = std::strcmp(v, "mixed") == 0 ? GPP_PREC_MIXED
-                                                   : GPP_PREC_FP64;
+      opt.precision = std::strcmp(v, "mixed") == 0      ? GPP_PREC_MIXED
+                      : std::strcmp(v, "factored") == 0 ? GPP_PREC_FP64_FACTORED
+                                                        : GPP_PREC_FP64;
     } else if (const char *v = val("--semantics=")) {
       opt.semantics = std::strcmp(v, "bgw") == 0      ? GPP_SEM_BGW
                       : std::strcmp(v, "strict") == 0 ? GPP_SEM_GPPKER_STRICT

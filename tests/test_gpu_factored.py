@@ -3,7 +3,7 @@
 sum P * delw * I_eps.  Same bar as the per-term FP64 kernel: within 1e-11
 relative of the oracle per achtemp/asxtemp entry, on ragged shapes (bands not
 a multiple of the 4-band matrix step, G' rows not a multiple of 64, ncouls
-not a multiple of the 128 G-vector tile), skipped G' rows, exact resonances,
+not a multiple of the 64 G-vector tile), skipped G' rows, exact resonances,
 every semantics, host and device inputs.
 """
 import numpy as np

@@ -425,3 +425,28 @@ def test_shape_sweep_vs_oracle(handle, shape, config):
     inp = gppk.fill_inputs(config, nb, nc, npg, nvband=nv, seed=nb + 7 * nc)
     for sem in ("gppker", "strict", "bgw"):
         check(handle.sigma(inp, semantics=sem), orc.sigma(inp, sem))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "factored"])
+def test_cli_matches_reference_binary_lines(precision):
+    """`gpp nb nv nc npg` (the mini-app's constant fills) prints the achtemp
+    lines of the reference binary (tests/golden/ref_constfill.json, printed
+    at 15 digits after the reference's plain sum) and of the closed form."""
+    import json
+    import re
+    import subprocess
+    from pathlib import Path
+    golden = json.loads((Path(__file__).parent / "golden" / "ref_constfill.json").read_text())
+    for r in golden:
+        args = [str(r[k]) for k in ("number_bands", "nvband", "ncouls", "nodes_per_group")]
+        say("cli", args, precision)
+        out = subprocess.run([str(gppk.CLI_PATH)] + args + [f"--precision={precision}"],
+                             capture_output=True, text=True, timeout=120, check=True).stdout
+        got = {}
+        for m in re.finditer(r"^achtemp\[(\d)\] = \(([^,]+),([^)]+)\)$", out, re.M):
+            got[int(m.group(1))] = complex(float(m.group(2)), float(m.group(3)))
+        ref = np.array([complex(*z) for z in r["achtemp"]])
+        g = np.array([got[i] for i in range(3)])
+        assert rel_err(g, ref) < 5e-12, (args, g, ref)
+        ach, _ = closed_form(int(args[0]), int(args[1]), int(args[2]), int(args[3]), "gppker")
+        assert rel_err(g, ach) < 1e-13, (args, g, ach)
