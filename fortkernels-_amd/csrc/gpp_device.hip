@@ -922,12 +922,11 @@ __global__ __launch_bounds__(kOccThreads) void occ_kernel(
 // (A = aqsmG [n1][my_igp], B = aqsn [n1][ig]).  A block owns 64 G' columns
 // (one group, as the other kernels) by 64 G-vectors; each of its four waves
 // a 32 x 32 quarter: 2 x 2 tiles of v_mfma_f64_16x16x4 (16 x 16 outputs, 4
-// bands per step).  That leaves room for 3 waves per SIMD, which hide the
-// operand loads better than 2 waves of 32 x 64 quarters (3.8-3.9 ms against
-// 4.5-4.6 ms at 512/2/32768/20 on MI355X).  Operands come straight from
-// global memory:
-// in one step a lane reads one complex of A per row tile and one of B per
-// column tile (16 lanes cover 256 contiguous bytes of a band's row).  P
+// bands per step).  That leaves room for 3 waves per SIMD (2 waves of
+// 32 x 64 quarters were 15% slower).  The band rows of A and B go through
+// LDS 8 bands at a time, double-buffered, the next stage's global loads in
+// flight during this stage's MFMAs (reading the operands straight from
+// global memory instead was 10% slower at 512/2/32768/20 on MI355X).  P
 // stays in registers and the epilogue contracts it with delw * I_eps of
 // the same (my_igp, ig) read once from HBM; P is never stored.
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -936,6 +935,9 @@ constexpr int kGemmNB = 2;   // 16-wide column tiles per wave
 constexpr int kGemmN = 2 * 16 * kGemmNB;  // G-vectors per block
 constexpr int kGemmThreads = 256;
 constexpr int kGemmWaves = 3;  // waves per SIMD the registers are sized for
+constexpr int kGemmStage = 8;  // bands per LDS stage (a multiple of 4)
+static_assert(kGemmN == kWave && kGemmM == kWave,
+              "one lane per column of the staged A and B rows");
 
 __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -995,17 +997,6 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
   const int c0 = ig_tile * kGemmN + (wave >> 1) * 16 * kGemmNB;
   const int li = lane & 15, lk = lane >> 4;
 
-  // Operand columns of this lane.  Rows past NG and columns past NC are
-  // clamped to real data (finite) and get zero weight in the epilogue.
-  const double2 *ap[2];
-  const double2 *bp[kGemmNB];
-#pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
-    ap[mb] = p.aqsmG + min(m0 + 16 * mb + li, NG - 1);
-#pragma unroll
-  for (int nb = 0; nb < kGemmNB; ++nb)
-    bp[nb] = p.aqsn + min(c0 + 16 * nb + li, NC - 1);
-
   dbl4 pr[2][kGemmNB], pi[2][kGemmNB];
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
@@ -1015,50 +1006,66 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
       pi[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
 
-  const int nsteps = (n1_end - n1_begin) / 4;
-  const size_t sa = (size_t)4 * NG, sb = (size_t)4 * NC;
-  size_t oa = (size_t)(n1_begin + lk) * NG, ob = (size_t)(n1_begin + lk) * NC;
-  double2 a[2], bv[kGemmNB];
-  if (nsteps > 0) {
+  // Bands go through LDS kGemmStage at a time, double-buffered.  Thread
+  // (wave, lane) copies rows wave*kRows .. +kRows of the stage: one complex
+  // of aqsmG (G' column lane of the block) and one of aqsn (G-vector lane),
+  // so each row is 1 KB read by one wave instruction.  The next stage's
+  // loads are issued into registers before this stage's MFMAs and written
+  // to the other buffer after them.  Rows past n1_end are stored as zeros;
+  // columns past NG / NC read clamped (finite) data and get zero weight in
+  // the epilogue.
+  constexpr int kRows = kGemmStage / 4;
+  __shared__ double2 As[2][kGemmStage][kGemmM];
+  __shared__ double2 Bs[2][kGemmStage][kGemmN];
+  const int la = min(igp_grp * kGemmM + lane, NG - 1);
+  const int lb = min(ig_tile * kGemmN + lane, NC - 1);
+  const int nst = (n1_end - n1_begin + kGemmStage - 1) / kGemmStage;
+  double2 ga[kRows], gb[kRows];
+  auto gload = [&](int st) {
 #pragma unroll
-    for (int mb = 0; mb < 2; ++mb) a[mb] = ap[mb][oa];
-#pragma unroll
-    for (int nb = 0; nb < kGemmNB; ++nb) bv[nb] = bp[nb][ob];
-  }
-  for (int s = 0; s < nsteps; ++s) {
-    // Next step's operands (the last iteration reloads its own step: in
-    // bounds, and no branch in the loop).
-    const bool more = s + 1 < nsteps;
-    oa += more ? sa : 0;
-    ob += more ? sb : 0;
-    double2 an[2], bn[kGemmNB];
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) an[mb] = ap[mb][oa];
-#pragma unroll
-    for (int nb = 0; nb < kGemmNB; ++nb) bn[nb] = bp[nb][ob];
-    gemm_step(a, bv, pr, pi);
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) a[mb] = an[mb];
-#pragma unroll
-    for (int nb = 0; nb < kGemmNB; ++nb) bv[nb] = bn[nb];
-  }
-  // Remaining 1-3 bands: lanes whose band is past n1_end contribute zeros.
-  const int n1t = n1_begin + 4 * nsteps + lk;
-  if (n1_begin + 4 * nsteps < n1_end) {
-    const bool in = n1t < n1_end;
-    const size_t ta = (size_t)min(n1t, n1_end - 1) * NG;
-    const size_t tb = (size_t)min(n1t, n1_end - 1) * NC;
-#pragma unroll
-    for (int mb = 0; mb < 2; ++mb) {
-      const double2 v = ap[mb][ta];
-      a[mb] = in ? v : make_double2(0.0, 0.0);
+    for (int r = 0; r < kRows; ++r) {
+      const int n1 = n1_begin + st * kGemmStage + wave * kRows + r;
+      const int n1c = min(n1, n1_end - 1);
+      const double2 va = p.aqsmG[(size_t)n1c * NG + la];
+      const double2 vb = p.aqsn[(size_t)n1c * NC + lb];
+      const bool in = n1 < n1_end;
+      ga[r] = in ? va : make_double2(0.0, 0.0);
+      gb[r] = in ? vb : make_double2(0.0, 0.0);
     }
+  };
+  auto lstore = [&](int buf) {
 #pragma unroll
-    for (int nb = 0; nb < kGemmNB; ++nb) {
-      const double2 v = bp[nb][tb];
-      bv[nb] = in ? v : make_double2(0.0, 0.0);
+    for (int r = 0; r < kRows; ++r) {
+      As[buf][wave * kRows + r][lane] = ga[r];
+      Bs[buf][wave * kRows + r][lane] = gb[r];
     }
-    gemm_step(a, bv, pr, pi);
+  };
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 16 * kGemmNB;
+  if (nst > 0) {
+    gload(0);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nst;
+    if (more) gload(st + 1);
+    // Keeps the next stage's loads issued ahead of this stage's MFMAs
+    // (otherwise the compiler moves them down to their LDS stores).
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int kk = 0; kk < kGemmStage / 4; ++kk) {
+      const int kr = 4 * kk + lk;
+      double2 a[2], bv[kGemmNB];
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) a[mb] = As[buf][kr][wm + 16 * mb + li];
+#pragma unroll
+      for (int nb = 0; nb < kGemmNB; ++nb)
+        bv[nb] = Bs[buf][kr][wn + 16 * nb + li];
+      gemm_step(a, bv, pr, pi);
+    }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
   }
 
   // Epilogue.  Output (i, j) of a 16 x 16 tile sits in lane 16*(i%4) + j,
