@@ -559,7 +559,7 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   if (run_occ)
     HIP_OR_RET(h, gpp::launch_occ(dp, n1o_end, o->semantics,
                                   part + (size_t)n_unocc * gpp::kSlots, n1o,
-                                  s));
+                                  s, factored));
   HIP_OR_RET(h, hipEventRecord(h->ev[3], s));
   if (n_ach)
     HIP_OR_RET(h, gpp::launch_achstemp(dp, o->semantics,

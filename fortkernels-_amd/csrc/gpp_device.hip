@@ -832,7 +832,10 @@ __device__ __forceinline__ void occ_term(const OccG &g, const OccBand &B,
   }
 }
 
-template <int SEM, bool N1LOG, typename Prob>
+// FACT: every band has the same frequencies (wx shared), so the solver's
+// sch and ssx do not depend on n1; the bands' matngmatmgp are summed first
+// and the solver runs once per G-vector instead of once per band.
+template <int SEM, bool N1LOG, typename Prob, bool FACT = false>
 __device__ __forceinline__ void occ_block(const Prob &p, int n1_end,
                                           int b, double *__restrict__ rec,
                                           double *__restrict__ n1log) {
@@ -877,6 +880,19 @@ __device__ __forceinline__ void occ_block(const Prob &p, int n1_end,
       n1log_store(wsch * bsr[2], wsch * bsi[2], n1log,
                   (size_t)b * n1_end + n1);
     }
+  } else if (FACT) {
+    const OccBand B = occ_band(p, 0);
+    for (int ig = ig_begin; ig < ig_end; ig += blockDim.x) {
+      const OccG g = occ_g<SEM>(p, my_igp, ig);
+      double mr = 0.0, mi = 0.0;
+      for (int n1 = 0; n1 < n1_end; ++n1) {
+        const double2 am = ld(p.aqsm + (size_t)n1 * NC + igp);
+        const double2 an = ld(p.aqsn + (size_t)n1 * NC + ig);
+        mr += an.x * am.x + an.y * am.y;
+        mi += an.y * am.x - an.x * am.y;
+      }
+      occ_term<SEM>(g, B, mr, mi, tsr, tsi, txr, txi);
+    }
   } else {
     for (int ig = ig_begin; ig < ig_end; ig += blockDim.x) {
       const OccG g = occ_g<SEM>(p, my_igp, ig);
@@ -901,12 +917,12 @@ __device__ __forceinline__ void occ_block(const Prob &p, int n1_end,
   block_store<12>(v, rec);
 }
 
-template <int SEM, bool N1LOG>
+template <int SEM, bool N1LOG, bool FACT = false>
 __global__ __launch_bounds__(kOccThreads) void occ_kernel(
     DevProblem p, int n1_end, double *__restrict__ partials,
     double *__restrict__ n1log) {
-  occ_block<SEM, N1LOG>(p, n1_end, blockIdx.x,
-                        partials + (size_t)blockIdx.x * kSlots, n1log);
+  occ_block<SEM, N1LOG, DevProblem, FACT>(
+      p, n1_end, blockIdx.x, partials + (size_t)blockIdx.x * kSlots, n1log);
 }
 
 // ---------------------------------------------------------------------------
@@ -1561,28 +1577,34 @@ int occ_grid(const DevProblem &p) {
 
 template <int SEM>
 static void launch_occ_sem(const DevProblem &p, int n1_end, double *partials,
-                           double *n1log, int grid, hipStream_t s) {
+                           double *n1log, bool fact, int grid, hipStream_t s) {
   if (n1log)
     occ_kernel<SEM, true><<<grid, kOccThreads, 0, s>>>(p, n1_end, partials,
                                                        n1log);
+  else if (fact)
+    occ_kernel<SEM, false, true><<<grid, kOccThreads, 0, s>>>(
+        p, n1_end, partials, n1log);
   else
     occ_kernel<SEM, false><<<grid, kOccThreads, 0, s>>>(p, n1_end, partials,
                                                         n1log);
 }
 
 hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
-                      double *partials, double *n1log, hipStream_t s) {
+                      double *partials, double *n1log, hipStream_t s,
+                      bool shared_wx) {
   const int grid = occ_grid(p);
   if (grid == 0 || n1_end <= 0) return hipSuccess;
   switch (sem) {
     case kSemGppker:
-      launch_occ_sem<kSemGppker>(p, n1_end, partials, n1log, grid, s);
+      launch_occ_sem<kSemGppker>(p, n1_end, partials, n1log, shared_wx, grid,
+                                 s);
       break;
     case kSemGppkerStrict:
-      launch_occ_sem<kSemGppkerStrict>(p, n1_end, partials, n1log, grid, s);
+      launch_occ_sem<kSemGppkerStrict>(p, n1_end, partials, n1log, shared_wx,
+                                       grid, s);
       break;
     case kSemBgw:
-      launch_occ_sem<kSemBgw>(p, n1_end, partials, n1log, grid, s);
+      launch_occ_sem<kSemBgw>(p, n1_end, partials, n1log, shared_wx, grid, s);
       break;
     default:
       return hipErrorInvalidValue;

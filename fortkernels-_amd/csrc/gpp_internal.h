@@ -91,8 +91,11 @@ hipError_t launch_unocc_gemm(const DevProblem &p, int n1_begin, int n1_end,
                              int grp_lo = 0, int grp_cnt = -1);
 int occ_grid(const DevProblem &p);
 // n1log: NULL, or [grid][n1_end] complex per-band records.
+// shared_wx: every band has the same frequencies; the bands' products are
+// then summed before the solver, which runs once per (my_igp, ig) (no n1log).
 hipError_t launch_occ(const DevProblem &p, int n1_end, int sem,
-                      double *partials, double *n1log, hipStream_t s);
+                      double *partials, double *n1log, hipStream_t s,
+                      bool shared_wx = false);
 hipError_t launch_reduce(const double *partials, int nrec, double *out,
                          hipStream_t s);
 hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,

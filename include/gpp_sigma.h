@@ -57,9 +57,11 @@ typedef struct { double re, im; } gpp_c128;
 #define GPP_PREC_FP64_FACTORED 2 /* FP64 with the band sum taken first:
                                 P[my_igp][ig] = sum_n1 conj(aqsm[n1][igp]) *
                                 aqsn[n1][ig] on the FP64 matrix cores, then
-                                sum P * delw * I_eps.  Needs wx shared by
-                                all bands (wx_per_band = 0); no
-                                acht_n1_loc, batches or tile tuning       */
+                                sum P * delw * I_eps; the occupied bands'
+                                products are likewise summed before their
+                                solver.  Needs wx shared by all bands
+                                (wx_per_band = 0); no acht_n1_loc,
+                                batches or tile tuning                    */
 
 /* Which variant of the reference physics to follow. */
 #define GPP_SEM_GPPKER         0 /* gppKer.cpp: unmasked non-occupied sch;
