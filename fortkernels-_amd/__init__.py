@@ -252,6 +252,45 @@ def make_problem(inp: dict) -> Problem:
                    _ptr(wx), per_band)
 
 
+class BatchProblems:
+    """The C problem table of a batch, built once from a list of input
+    dicts.  Passed to Handle.batch in place of the list it skips rebuilding
+    the table on every call; the arrays must stay alive and in place (their
+    contents may change between calls)."""
+
+    def __init__(self, inputs: list):
+        self.inputs = list(inputs)
+        self.table = (Problem * len(self.inputs))(
+            *[make_problem(d) for d in self.inputs])
+
+    def __len__(self) -> int:
+        return len(self.inputs)
+
+
+# numpy view of gpp_result (include/gpp_sigma.h), for converting a batch's
+# results in bulk.
+_RESULT_DTYPE = np.dtype([
+    ("achtemp", np.complex128, (3,)), ("asxtemp", np.complex128, (3,)),
+    ("achstemp", np.complex128), ("acht_n1_loc", np.uintp),
+    ("unocc_kernel_ms", np.float64), ("occ_kernel_ms", np.float64),
+    ("total_ms", np.float64), ("terms", np.float64),
+    ("fp64_fallback_terms", np.int64)])
+
+
+def _batch_results(res, n: int) -> list:
+    if _RESULT_DTYPE.itemsize != C.sizeof(Result):
+        raise RuntimeError("gpp_result layout mismatch")
+    a = np.frombuffer(res, dtype=_RESULT_DTYPE, count=n)
+    ach, asx = a["achtemp"].copy(), a["asxtemp"].copy()
+    achs = a["achstemp"].tolist()
+    rest = [a[k].tolist() for k in ("unocc_kernel_ms", "occ_kernel_ms",
+                                    "total_ms", "terms", "fp64_fallback_terms")]
+    return [{"achtemp": ach[i], "asxtemp": asx[i], "achstemp": achs[i],
+             "unocc_kernel_ms": rest[0][i], "occ_kernel_ms": rest[1][i],
+             "total_ms": rest[2][i], "terms": rest[3][i],
+             "fp64_fallback_terms": rest[4][i]} for i in range(n)]
+
+
 def _cplx(z: c128) -> complex:
     return complex(z.re, z.im)
 
@@ -365,19 +404,22 @@ class Handle:
             outs[k] = getattr(res, k)
         return outs
 
-    def batch(self, inputs: list, precision="fp64", semantics="gppker",
+    def batch(self, inputs, precision="fp64", semantics="gppker",
               memory=None, flags=0, stream=None, resonance_threshold=0.0,
               tile_ig=0) -> list:
         """Many independent evaluations in one batched launch sequence
-        (gpp_sigma_batch), replayed from a hipGraph while unchanged."""
-        probs = (Problem * len(inputs))(*[make_problem(d) for d in inputs])
-        opt = self._options(inputs[0], precision, semantics, memory, flags,
+        (gpp_sigma_batch), replayed from a hipGraph while unchanged.
+        `inputs` is a list of input dicts or a BatchProblems built from one
+        (which skips rebuilding the problem table on every call)."""
+        bp = inputs if isinstance(inputs, BatchProblems) else BatchProblems(inputs)
+        n = len(bp)
+        opt = self._options(bp.inputs[0], precision, semantics, memory, flags,
                             stream, resonance_threshold, tile_ig, 0)
-        res = (Result * len(inputs))()
-        rc = self.lib.gpp_sigma_batch(self.h, probs, len(inputs), C.byref(opt), res)
+        res = (Result * n)()
+        rc = self.lib.gpp_sigma_batch(self.h, bp.table, n, C.byref(opt), res)
         if rc != GPP_OK:
             raise GppError(rc, self.last_error())
-        return [self._result(r) for r in res]
+        return _batch_results(res, n)
 
     def autotune(self, inp: dict, precision="fp64", semantics="gppker",
                  memory=None, stream=None) -> tuple:

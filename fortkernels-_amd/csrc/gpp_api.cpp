@@ -375,6 +375,46 @@ void fill_result(const double *ho, const double *hach, gpp_result *r) {
   r->achstemp.im = hach[1];
 }
 
+// Replays the captured batch graph on s and returns its results.  h->part
+// holds [counter 256 B][nrec_total records][n x 2 result slots], as
+// gpp_sigma_batch lays it out; ev[0] has been recorded on s.
+int finish_batch(gpp_handle h, const gpp_problem *p, int n,
+                 const gpp_options *o, bool want_ach, int nrec_total,
+                 hipStream_t s, gpp_result *r) {
+  unsigned long long *dcount = counter_ptr(h);
+  const double *part = (const double *)((char *)h->part.ptr + 256);
+  const double *dout = part + (size_t)nrec_total * gpp::kSlots;
+  HIP_OR_RET(h, hipGraphLaunch(h->graph_exec, s));
+  HIP_OR_RET(h, hipEventRecord(h->ev[1], s));
+  double *ho = h->host_out;
+  HIP_OR_RET(h, hipMemcpyAsync(ho, dout, (size_t)2 * n * gpp::kSlots * 8,
+                               hipMemcpyDeviceToHost, s));
+  HIP_OR_RET(h, hipMemcpyAsync(ho + (size_t)2 * n * gpp::kSlots, dcount, 8,
+                               hipMemcpyDeviceToHost, s));
+  HIP_OR_RET(h, hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+  long long fb = 0;
+  std::memcpy(&fb, ho + (size_t)2 * n * gpp::kSlots, 8);
+  for (int i = 0; i < n; ++i) {
+    fill_result(ho + (size_t)i * gpp::kSlots,
+                ho + (size_t)(n + i) * gpp::kSlots, r + i);
+    if (!want_ach) r[i].achstemp.re = r[i].achstemp.im = 0.0;
+    r[i].unocc_kernel_ms = ms;
+    r[i].occ_kernel_ms = 0.0;
+    r[i].total_ms = ms;
+    r[i].terms = (double)p[i].number_bands * p[i].ngpown *
+                 (double)p[i].ncouls * 3.0;
+    r[i].fp64_fallback_terms = (i == 0) ? fb : 0;
+    if (o->flags & GPP_FLAG_CHECK_FINITE)
+      for (int j = 0; j < 12; ++j)
+        if (!std::isfinite(ho[(size_t)i * gpp::kSlots + j]))
+          return set_err(h, GPP_ENONFINITE,
+                         "non-finite result in problem %d slot %d", i, j);
+  }
+  return GPP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -706,6 +746,31 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   // use the tile's waves per block.
   const TileCfg tile = pick_tile(h, p + big, o);
 
+  // Device inputs and the same call as the one the cached graph was captured
+  // for (problems, options, tile, buffers): replay it as it is.  The graph
+  // reads the caller's arrays on every replay, so new contents are picked
+  // up; host inputs are copied on every call and always take the full path.
+  std::vector<unsigned char> key;
+  if (o->memory == GPP_MEM_DEVICE) {
+    auto put = [&](const void *v, size_t b) {
+      const unsigned char *c = (const unsigned char *)v;
+      key.insert(key.end(), c, c + b);
+    };
+    key.reserve(sizeof(gpp_problem) * n + 128);
+    put(&n, sizeof n);
+    put(o, sizeof *o);
+    put(&tile, sizeof tile);
+    put(p, sizeof(gpp_problem) * n);
+  }
+  void *bases_now[4] = {h->part.ptr, h->meta.ptr, h->work.ptr, (void *)s};
+  if (!key.empty() && h->graph_exec && h->batch_key == key &&
+      std::memcmp(bases_now, h->batch_bases, sizeof bases_now) == 0 &&
+      h->host_out_cap >= (size_t)2 * n * gpp::kSlots + 1) {
+    HIP_OR_RET(h, hipEventRecord(h->ev[0], s));
+    return finish_batch(h, p, n, o, want_ach, h->batch_nrec, s, r);
+  }
+  h->batch_key.clear();
+
   std::vector<Prepared> prep;
   if ((rc = prepare(h, p, n, o, mixed, s, prep))) return rc;
 
@@ -870,35 +935,12 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     HIP_OR_RET(h, ie);
     h->graph_sig.swap(sig);
   }
-  HIP_OR_RET(h, hipGraphLaunch(h->graph_exec, s));
-  HIP_OR_RET(h, hipEventRecord(h->ev[1], s));
-  double *ho = h->host_out;
-  HIP_OR_RET(h, hipMemcpyAsync(ho, dout, (size_t)2 * n * gpp::kSlots * 8,
-                               hipMemcpyDeviceToHost, s));
-  HIP_OR_RET(h, hipMemcpyAsync(ho + (size_t)2 * n * gpp::kSlots, dcount, 8,
-                               hipMemcpyDeviceToHost, s));
-  HIP_OR_RET(h, hipStreamSynchronize(s));
-  float ms = 0.f;
-  HIP_OR_RET(h, hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
-  long long fb = 0;
-  std::memcpy(&fb, ho + (size_t)2 * n * gpp::kSlots, 8);
-  for (int i = 0; i < n; ++i) {
-    fill_result(ho + (size_t)i * gpp::kSlots,
-                ho + (size_t)(n + i) * gpp::kSlots, r + i);
-    if (!want_ach) r[i].achstemp.re = r[i].achstemp.im = 0.0;
-    r[i].unocc_kernel_ms = ms;
-    r[i].occ_kernel_ms = 0.0;
-    r[i].total_ms = ms;
-    r[i].terms = (double)p[i].number_bands * p[i].ngpown *
-                 (double)p[i].ncouls * 3.0;
-    r[i].fp64_fallback_terms = (i == 0) ? fb : 0;
-    if (o->flags & GPP_FLAG_CHECK_FINITE)
-      for (int j = 0; j < 12; ++j)
-        if (!std::isfinite(ho[(size_t)i * gpp::kSlots + j]))
-          return set_err(h, GPP_ENONFINITE,
-                         "non-finite result in problem %d slot %d", i, j);
+  if (!key.empty()) {
+    h->batch_key.swap(key);
+    std::memcpy(h->batch_bases, bases, sizeof bases);
+    h->batch_nrec = nrec + nach;
   }
-  return GPP_OK;
+  return finish_batch(h, p, n, o, want_ach, nrec + nach, s, r);
 }
 
 int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,

@@ -85,6 +85,12 @@ struct gpp_handle_s {
   // Cached graph of the last batch (replayed when the call is identical).
   hipGraphExec_t graph_exec = nullptr;
   std::vector<unsigned char> graph_sig;
+  // Device-memory batches: the call (problems, options, tile) and buffers
+  // the graph was captured for, and its record count, so that a repeated
+  // call replays it without planning again.
+  std::vector<unsigned char> batch_key;
+  void *batch_bases[4] = {nullptr, nullptr, nullptr, nullptr};
+  int batch_nrec = 0;
   std::string err;
   gpp::Stager *stager = nullptr;  // host-input copies (created on first use)
 };

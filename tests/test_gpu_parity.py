@@ -365,6 +365,35 @@ def test_batch_device_memory(handle):
         check(r, orc.sigma(host))
 
 
+def test_batch_device_replay_reads_new_contents(handle):
+    """A repeated device-memory batch replays its captured graph without
+    planning again; the graph reads the caller's arrays on every replay, so
+    contents changed in place between calls are used, also after an eager
+    call on the same handle that grows its buffers."""
+    import torch
+    say("batch device replay")
+    insts = [gppk.fill_inputs_device(3, nb, nc, 8, nvband=nv, seed=400 + i)
+             for i, (nb, nc, nv) in enumerate([(5, 300, 2), (8, 640, 1), (3, 1000, 0)])]
+    torch.cuda.synchronize()
+    first = handle.batch(insts, memory=gppk.MEM_DEVICE)
+    again = handle.batch(insts, memory=gppk.MEM_DEVICE)
+    for a, b in zip(first, again):
+        assert np.array_equal(a["achtemp"], b["achtemp"])
+    insts[1]["aqsntemp"].mul_(2.0)
+    insts[2]["I_eps_array"].mul_(0.5)
+    torch.cuda.synchronize()
+    res = handle.batch(insts, memory=gppk.MEM_DEVICE)
+    for d, r in zip(insts, res):
+        host = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in d.items()}
+        check(r, orc.sigma(host))
+    big = gppk.fill_inputs(2, 40, 4096, 4, nvband=2, seed=9)
+    handle.sigma(big)
+    res = handle.batch(insts, memory=gppk.MEM_DEVICE)
+    for d, r in zip(insts, res):
+        host = {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in d.items()}
+        check(r, orc.sigma(host))
+
+
 @pytest.mark.parametrize("sem", ["gppker", "bgw"])
 def test_occupied_cutoff_closed_form(handle, sem):
     """The bgw cutoff (|ssx| > 4 |I_eps| and wx < 0) on band-resolved

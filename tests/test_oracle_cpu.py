@@ -410,3 +410,25 @@ def test_oracle_acht_n1_loc_is_each_bands_achtemp2(sem):
     for n1 in range(5):
         one = orc.sigma(gppk.band_slice(inp, n1, n1 + 1), sem)["achtemp"][2]
         assert abs(full[n1] - one) <= 1e-13 * abs(one), (n1, full[n1], one)
+
+
+def test_batch_result_view_matches_struct_fields():
+    """The bulk numpy view of a gpp_result array used for batch results
+    reads the same values as the field-by-field ctypes conversion."""
+    import ctypes as C
+    res = (gppk.Result * 3)()
+    for i in range(3):
+        for iw in range(3):
+            res[i].achtemp[iw].re, res[i].achtemp[iw].im = i + iw, -iw - 0.5
+            res[i].asxtemp[iw].re, res[i].asxtemp[iw].im = 2.0 * iw, i * 0.25
+        res[i].achstemp.re, res[i].achstemp.im = 1.0 + i, -3.0
+        res[i].unocc_kernel_ms, res[i].occ_kernel_ms = 1.5 * i, 0.5
+        res[i].total_ms, res[i].terms = 2.5, 1e9 + i
+        res[i].fp64_fallback_terms = 7 * i
+    fast = gppk._batch_results(res, 3)
+    for i in range(3):
+        ref = gppk.Handle._result(res[i])
+        assert set(fast[i]) == set(ref)
+        for k, v in ref.items():
+            assert np.array_equal(np.asarray(fast[i][k]), np.asarray(v)), (i, k)
+    assert gppk._RESULT_DTYPE.itemsize == C.sizeof(gppk.Result)
