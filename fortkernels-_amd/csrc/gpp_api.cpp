@@ -959,6 +959,8 @@ int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
                    0};
   gpp_options o;
   gpp_default_options(&o);
+  // One wx[3] for all bands: the band-sum-first FP64 evaluation applies.
+  o.precision = GPP_PREC_FP64_FACTORED;
   gpp_result res;
   std::memset(&res, 0, sizeof res);
   rc = gpp_sigma_ex(h, &p, &o, &res);
@@ -990,6 +992,7 @@ int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
                    0};
   gpp_options o;
   gpp_default_options(&o);
+  o.precision = GPP_PREC_FP64_FACTORED;  // one wx[3] for all bands
   o.flags = GPP_FLAG_ALL_BANDS_UNOCC | GPP_FLAG_NO_ACHSTEMP;
   gpp_result res;
   std::memset(&res, 0, sizeof res);

@@ -180,8 +180,10 @@ GPP_API int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *
  * Drop-in for the band loop of gppKer.cpp:270-361.  Host pointers,
  * FP64, GPP_SEM_GPPKER, a library-owned handle on the current device.
  * wx_array holds 3 values shared by all bands, as gppKer.cpp:276-280
- * computes them; |wx| < 1e-6 is clamped to 1e-6 as there.
- * achtemp and asxtemp receive 3 values each.
+ * computes them; |wx| < 1e-6 is clamped to 1e-6 as there.  Since the
+ * frequencies are shared, the sum is evaluated band sum first
+ * (GPP_PREC_FP64_FACTORED: the same terms in FP64, another summation
+ * order).  achtemp and asxtemp receive 3 values each.
  */
 GPP_API int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
               const gpp_c128 *aqsmtemp, const gpp_c128 *aqsntemp,
@@ -193,7 +195,8 @@ GPP_API int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
 /*
  * Drop-ins for the reference's operator split (gppKer_gpuComplex.cpp).
  * gpp_achtemp_kernel replaces achtemp_kernel (:148): every band taken as
- * non-occupied, results added into the six out-references.
+ * non-occupied (band sum first, its wx_array being shared by all bands),
+ * results added into the six out-references.
  * gpp_tillNvband_kernel replaces tillNvband_kernel (:127): bands
  * n1 < nvband, asxtemp[iw] += ssxt * occ * vcoul[igp].
  * Both take host pointers and run FP64 on the current device.

@@ -186,6 +186,11 @@ def test_dropin_entry_points(handle):
     ref = orc.sigma(inp)
     check({"achtemp": np.array([complex(z.re, z.im) for z in ach]),
            "asxtemp": np.array([complex(z.re, z.im) for z in asx])}, ref)
+    # the drop-in evaluates band sum first (its wx_array is shared by all
+    # bands): the same bits as gpp_sigma_ex with that precision
+    ex = handle.sigma(inp, precision="fp64_factored")
+    assert np.array_equal(ex["achtemp"], np.array([complex(z.re, z.im) for z in ach]))
+    assert np.array_equal(ex["asxtemp"], np.array([complex(z.re, z.im) for z in asx]))
     # achtemp_kernel: all bands unoccupied, adds into the six references
     d = [ctypes.c_double(1.0) for _ in range(6)]
     rc = lib.gpp_achtemp_kernel(nc, ng, P("wx_array"), P("wtilde_array"), P("aqsmtemp"),
