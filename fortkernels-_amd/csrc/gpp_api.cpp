@@ -513,10 +513,6 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   const bool want_n1 = r->acht_n1_loc != nullptr;
 
   const bool factored = o->precision == GPP_PREC_FP64_FACTORED;
-  if (factored && want_n1)
-    return set_err(h, GPP_EUNSUPPORTED,
-                   "acht_n1_loc needs per-band sums; the factored band sum "
-                   "has none");
   if ((o->flags & GPP_FLAG_AUTOTUNE) && !factored && !o->tile_ig &&
       !o->waves_per_block && !h->tuned.count(tune_key(p, o))) {
     int k, w;
@@ -553,8 +549,13 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   double *part = (double *)((char *)h->part.ptr + 256);
   double *dout = part + (size_t)(nrec + n_ach) * gpp::kSlots;
   double *n1u = nullptr, *n1o = nullptr, *n1out = nullptr;
+  // Per-band log records of the non-occupied bands: one per block of the
+  // per-term kernels, one per (group, G-vector chunk) in the factored mode.
+  const int n_log_u = !(run_unocc && want_n1) ? 0
+                      : factored ? gpp::n1log_gemm_records(dp)
+                                 : n_unocc;
   if (want_n1) {
-    const size_t nu = run_unocc ? (size_t)n_unocc * (NB - n1u_begin) : 0;
+    const size_t nu = run_unocc ? (size_t)n_log_u * (NB - n1u_begin) : 0;
     const size_t no = run_occ ? (size_t)n_occ * n1o_end : 0;
     if ((rc = ensure(h, h->n1log, (nu + no + NB + 1) * 16))) return rc;
     n1u = (double *)h->n1log.ptr;
@@ -586,9 +587,13 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
                                p->wtilde_array + off, len));
         HIP_OR_RET(h, st->fence(s));
       }
-      if (factored)
+      if (factored) {
         HIP_OR_RET(h, gpp::launch_unocc_gemm(dp, n1u_begin, NB, o->semantics,
                                              part, s, g, cnt));
+        if (want_n1)
+          HIP_OR_RET(h, gpp::launch_n1log_gemm(dp, n1u_begin, NB,
+                                               o->semantics, n1u, s, g, cnt));
+      }
       else
         HIP_OR_RET(h, gpp::launch_unocc(dp, n1u_begin, NB, o->precision,
                                         o->semantics, tile, thr, part, n1u,
@@ -608,7 +613,7 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   HIP_OR_RET(h, gpp::launch_reduce(part + (size_t)nrec * gpp::kSlots, n_ach,
                                    dout + gpp::kSlots, s));
   if (want_n1)
-    HIP_OR_RET(h, gpp::launch_reduce_n1(n1u, run_unocc ? n_unocc : 0,
+    HIP_OR_RET(h, gpp::launch_reduce_n1(n1u, run_unocc ? n_log_u : 0,
                                         n1u_begin,
                                         run_unocc ? NB - n1u_begin : 0, n1o,
                                         run_occ ? n_occ : 0, n1o_end, NB,

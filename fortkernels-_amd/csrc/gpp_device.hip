@@ -1151,6 +1151,208 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
 
 }
 
+// Per-band log (acht_n1_loc, gppKer.cpp:357) of the band-sum-first mode.
+// For a non-occupied band n1 the log is
+//   acht_n1_loc[n1] = sum_{my_igp} 0.5 vcoul[igp] conj(aqsm[n1][igp]) Q[n1][my_igp]
+//   Q[n1][my_igp]   = sum_ig aqsn[n1][ig] * S2[my_igp][ig]
+// with S2 = wtilde*I_eps*conj(wdiff_2)/|wdiff_2|^2, the third frequency's
+// factor of the sum (masked as the bgw semantics masks it).  Q is a complex
+// matrix product over the G-vectors, formed on the FP64 matrix cores like P
+// (Re Q = Ar Br - Ai Bi, Im Q = Ar Bi + Ai Br).  A block owns 64 bands by
+// 64 G' columns (one group) and one chunk of kLogChunk G-vectors; its four
+// waves hold 32 x 32 quarters.  Rows of aqsn and of S2 (computed from
+// wtilde and I_eps while staging) go through LDS kLogStage G-vectors at a
+// time.  The epilogue contracts Q with the weights and conj(aqsm) over the
+// block's G' columns and stores one complex per band into the record of
+// (group, chunk): [record][n1 - n1_begin], the layout launch_reduce_n1 reads.
+constexpr int kLogM = 64;        // bands per block
+constexpr int kLogStage = 8;     // G-vectors per LDS stage
+constexpr int kLogChunk = 2048;  // G-vectors per block
+
+__device__ __forceinline__ void log_step(const double2 (&a)[2],
+                                         const double2 (&b)[2],
+                                         dbl4 (&qr)[2][2], dbl4 (&qi)[2][2]) {
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      qr[mb][nb] = mfma4(a[mb].x, b[nb].x, qr[mb][nb]);
+      qi[mb][nb] = mfma4(a[mb].x, b[nb].y, qi[mb][nb]);
+    }
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb) {
+    const double nai = -a[mb].y;
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      qr[mb][nb] = mfma4(nai, b[nb].y, qr[mb][nb]);
+      qi[mb][nb] = mfma4(a[mb].y, b[nb].x, qi[mb][nb]);
+    }
+  }
+}
+
+template <bool MASK>
+__global__ __launch_bounds__(kGemmThreads) __attribute__((
+    amdgpu_waves_per_eu(kGemmWaves))) void n1log_gemm_kernel(
+    DevProblem p, int n1_begin, int n1_end, int grp_lo,
+    double *__restrict__ n1log) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = uniform(threadIdx.x / kWave);
+  const int NC = p.NC, NG = p.NG;
+  const int nbu = n1_end - n1_begin;
+  const int nsplit = (NC + kLogChunk - 1) / kLogChunk;
+  const int bt = blockIdx.x;                 // band tile
+  const int grp = grp_lo + (int)blockIdx.y;  // G' group
+  const int ch = blockIdx.z;                 // G-vector chunk
+  const int i0 = bt * kLogM, m0 = grp * kGemmM;
+  const int k_begin = ch * kLogChunk, k_end = min(NC, k_begin + kLogChunk);
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+
+  dbl4 qr[2][2], qi[2][2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      qr[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      qi[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+    }
+
+  // Staging: thread t copies row t/4, G-vectors 2*(t%4) and 2*(t%4)+1 of
+  // the stage, of aqsn (bands) and of S2 (G' columns).  Rows are padded by
+  // one complex against LDS bank conflicts.
+  __shared__ double2 As[2][kLogM][kLogStage + 1];
+  __shared__ double2 Bs[2][kGemmM][kLogStage + 1];
+  const int sr = threadIdx.x >> 2, sc = (threadIdx.x & 3) * 2;
+  const int n1s = n1_begin + i0 + sr;
+  const bool a_row = i0 + sr < nbu;
+  const double2 *arow = p.aqsn + (size_t)min(n1s, n1_end - 1) * NC;
+  const int my_igp_s = m0 + sr;
+  const int igp_s = my_igp_s < NG ? p.igp_of[my_igp_s] : -1;
+  const double2 *wrow = p.wtilde + (size_t)min(my_igp_s, NG - 1) * NC;
+  const double2 *erow = p.ieps + (size_t)min(my_igp_s, NG - 1) * NC;
+  const double x2 = p.wx[n1_begin * 3 + 2];
+  // The loads of the next stage are issued before this stage's MFMAs; S2 is
+  // formed from them after the MFMAs, when they are stored to LDS.
+  double2 ga[2], gw[2], ge[2];
+  bool gin[2];
+  auto gload = [&](int kb) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int ig = kb + sc + c;
+      gin[c] = ig < k_end;
+      const int igc = min(ig, NC - 1);
+      ga[c] = arow[igc];
+      gw[c] = wrow[igc];
+      ge[c] = erow[igc];
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const bool live = gin[c] && igp_s >= 0;
+      As[buf][sr][sc + c] =
+          (gin[c] && a_row) ? ga[c] : make_double2(0.0, 0.0);
+      // S2 = A conj(wdiff_2) r_2, A = wtilde I_eps (0 for inert slots)
+      const double2 w = gw[c], e = ge[c];
+      const double wr = live ? w.x : 0.0, wi = live ? w.y : 0.0;
+      const double Ar = live ? w.x * e.x - w.y * e.y : 0.0;
+      const double Ai = live ? w.x * e.y + w.y * e.x : 0.0;
+      const double dr = x2 - wr;
+      const double wi2 = wi * wi;
+      const double d = fma(dr, dr, wi2);
+      double r = recip(d);
+      if (MASK)
+        r = (d > kLimitTwo && (wr * wr + wi2) * r < kLimitOne) ? r : 0.0;
+      Bs[buf][sr][sc + c] =
+          make_double2(fma(Ar, dr, -Ai * wi) * r, fma(Ai, dr, Ar * wi) * r);
+    }
+  };
+  const int nst = (k_end - k_begin + kLogStage - 1) / kLogStage;
+  if (nst > 0) {
+    gload(k_begin);
+    lstore(0);
+  }
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int buf = st & 1;
+    const bool more = st + 1 < nst;
+    if (more) gload(k_begin + (st + 1) * kLogStage);
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int kk = 0; kk < kLogStage / 4; ++kk) {
+      const int kr = 4 * kk + lk;
+      double2 a[2], b[2];
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) a[mb] = As[buf][wm + 16 * mb + li][kr];
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) b[nb] = Bs[buf][wn + 16 * nb + li][kr];
+      log_step(a, b, qr, qi);
+    }
+    if (more) lstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  // Epilogue: row i (band) of a tile sits in lanes 16*(i%4) + j, element
+  // i/4; the 16 lanes of a row hold its 16 G' columns.
+  __shared__ double red[2][kLogM][2];
+  double v[2][4][2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = wm + 16 * mb + lk + 4 * e;  // band within the tile
+      const int n1 = n1_begin + i0 + i;
+      const bool brow = i0 + i < nbu;
+      double sr2 = 0.0, si2 = 0.0;
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) {
+        const int my_igp = m0 + wn + 16 * nb + li;
+        const int igp = my_igp < NG ? p.igp_of[my_igp] : -1;
+        if (brow && igp >= 0) {
+          const double wgt = 0.5 * p.vcoul[igp];
+          const double2 am = p.aqsmG[(size_t)n1 * NG + my_igp];
+          const double Qr = qr[mb][nb][e], Qi = qi[mb][nb][e];
+          // conj(am) * Q
+          sr2 = fma(wgt, fma(am.x, Qr, am.y * Qi), sr2);
+          si2 = fma(wgt, fma(am.x, Qi, -am.y * Qr), si2);
+        }
+      }
+#pragma unroll
+      for (int m = 1; m < 16; m <<= 1) {
+        sr2 += __shfl_xor(sr2, m, kWave);
+        si2 += __shfl_xor(si2, m, kWave);
+      }
+      v[mb][e][0] = sr2;
+      v[mb][e][1] = si2;
+    }
+  // The two waves of a row half (wn = 0, 32) add in that order.
+  if (li == 0 && wn == 32) {
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = wm + 16 * mb + lk + 4 * e;
+        red[1][i][0] = v[mb][e][0];
+        red[1][i][1] = v[mb][e][1];
+      }
+  }
+  __syncthreads();
+  if (li == 0 && wn == 0) {
+    const size_t rec = (size_t)grp * nsplit + ch;
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int i = wm + 16 * mb + lk + 4 * e;
+        if (i0 + i < nbu) {
+          double *o = n1log + 2 * (rec * nbu + i0 + i);
+          o[0] = v[mb][e][0] + red[1][i][0];
+          o[1] = v[mb][e][1] + red[1][i][1];
+        }
+      }
+  }
+}
+
 // Per-band log reduction: out[n1] = sum over the unoccupied-kernel blocks
 // (records [rec][n1 - u_begin]) plus sum over the occupied-kernel blocks
 // (records [rec][n1]), each in record order.  One block per band.
@@ -1542,6 +1744,34 @@ hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
     default:
       return hipErrorInvalidValue;
   }
+}
+
+int n1log_gemm_records(const DevProblem &p) {
+  if (p.NG <= 0 || p.NC <= 0) return 0;
+  return unocc_groups(p) * ((p.NC + kLogChunk - 1) / kLogChunk);
+}
+
+hipError_t launch_n1log_gemm(const DevProblem &p, int n1_begin, int n1_end,
+                             int sem, double *n1log, hipStream_t s,
+                             int grp_lo, int grp_cnt) {
+  const int ngrp = unocc_groups(p);
+  if (grp_cnt < 0) {
+    grp_lo = 0;
+    grp_cnt = ngrp;
+  }
+  if (grp_lo < 0 || grp_lo + grp_cnt > ngrp) return hipErrorInvalidValue;
+  if (ngrp == 0 || grp_cnt == 0 || n1_begin >= n1_end) return hipSuccess;
+  const dim3 grid((n1_end - n1_begin + kLogM - 1) / kLogM, grp_cnt,
+                  (p.NC + kLogChunk - 1) / kLogChunk);
+  if (grid.x > 65535u * 1024u || grid.y > 65535u || grid.z > 65535u)
+    return hipErrorInvalidValue;
+  if (sem == kSemBgw)
+    n1log_gemm_kernel<true><<<grid, kGemmThreads, 0, s>>>(p, n1_begin, n1_end,
+                                                         grp_lo, n1log);
+  else
+    n1log_gemm_kernel<false><<<grid, kGemmThreads, 0, s>>>(
+        p, n1_begin, n1_end, grp_lo, n1log);
+  return hipGetLastError();
 }
 
 int unocc_gemm_grid(const DevProblem &p) {

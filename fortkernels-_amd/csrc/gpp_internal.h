@@ -89,6 +89,14 @@ int unocc_gemm_grid(const DevProblem &p);
 hipError_t launch_unocc_gemm(const DevProblem &p, int n1_begin, int n1_end,
                              int sem, double *partials, hipStream_t s,
                              int grp_lo = 0, int grp_cnt = -1);
+// Per-band log of the band-sum-first mode, non-occupied bands
+// [n1_begin, n1_end): records (group, G-vector chunk), each
+// [n1_end - n1_begin] complex, as launch_reduce_n1 reads them; a launch
+// over groups [grp_lo, grp_lo + grp_cnt) writes their records only.
+int n1log_gemm_records(const DevProblem &p);
+hipError_t launch_n1log_gemm(const DevProblem &p, int n1_begin, int n1_end,
+                             int sem, double *n1log, hipStream_t s,
+                             int grp_lo = 0, int grp_cnt = -1);
 int occ_grid(const DevProblem &p);
 // n1log: NULL, or [grid][n1_end] complex per-band records.
 // shared_wx: every band has the same frequencies; the bands' products are

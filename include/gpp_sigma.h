@@ -59,9 +59,10 @@ typedef struct { double re, im; } gpp_c128;
                                 aqsn[n1][ig] on the FP64 matrix cores, then
                                 sum P * delw * I_eps; the occupied bands'
                                 products are likewise summed before their
-                                solver.  Needs wx shared by all bands
-                                (wx_per_band = 0); no acht_n1_loc,
-                                batches or tile tuning                    */
+                                solver; acht_n1_loc by a second product
+                                over the G-vectors.  Needs wx shared by
+                                all bands (wx_per_band = 0); no batches
+                                or tile tuning                            */
 
 /* Which variant of the reference physics to follow. */
 #define GPP_SEM_GPPKER         0 /* gppKer.cpp: unmasked non-occupied sch;

@@ -137,11 +137,66 @@ def test_factored_rejects_what_it_cannot_do(handle):
     assert ei.value.code == gppk.GPP_EUNSUPPORTED
     inp = gppk.fill_inputs(2, 6, 256, 4, nvband=2, seed=5)
     with pytest.raises(gppk.GppError) as ei:
-        handle.sigma(inp, precision=FACT, acht_n1_loc=True)
-    assert ei.value.code == gppk.GPP_EUNSUPPORTED
-    with pytest.raises(gppk.GppError) as ei:
         handle.batch([inp, inp], precision=FACT)
     assert ei.value.code == gppk.GPP_EUNSUPPORTED
     with pytest.raises(gppk.GppError) as ei:
         handle.autotune(inp, precision=FACT)
     assert ei.value.code == gppk.GPP_EUNSUPPORTED
+
+
+@pytest.mark.parametrize("shape", [(10, 3, 700, 7), (70, 2, 2100, 10), (5, 0, 100, 3),
+                                   (131, 4, 4500, 30), (9, 9, 300, 4)])
+@pytest.mark.parametrize("sem", ["gppker", "strict", "bgw"])
+def test_factored_acht_n1_loc(handle, sem, shape):
+    """The per-band log in the factored mode (a second matrix product, over
+    the G-vectors, for the non-occupied bands; the occupied bands' solver per
+    band): against the oracle per band, several band tiles of 64 and
+    G-vector chunks of 2048; it sums to achtemp[2], and asking for it leaves
+    achtemp unchanged (bit for bit without occupied bands; with them their
+    solver then runs band by band, another summation order)."""
+    nb, nv, nc, npg = shape
+    say("factored acht_n1_loc", shape, sem)
+    inp = gppk.fill_inputs(2, nb, nc, npg, nvband=nv, seed=nb + nc)
+    got = handle.sigma(inp, precision=FACT, semantics=sem, acht_n1_loc=True)
+    ref = orc.sigma(inp, sem, want_acht_n1=True)
+    check(got, ref)
+    e = rel_err(got["acht_n1_loc"], ref["acht_n1_loc"])
+    assert e <= FP64_TOL, e
+    assert rel_err(np.sum(got["acht_n1_loc"]), got["achtemp"][2]) <= 1e-12
+    plain = handle.sigma(inp, precision=FACT, semantics=sem)
+    assert rel_err(got["achtemp"], plain["achtemp"]) <= 1e-13
+    if nv == 0:
+        assert np.array_equal(plain["achtemp"], got["achtemp"])
+
+
+def test_factored_acht_n1_loc_skipped_rows_and_host_inputs(handle):
+    """Skipped G' rows, and host inputs (chunked launches over G' groups)
+    giving the device inputs' bits for the per-band log."""
+    import torch
+    say("factored acht_n1_loc host")
+    inp = gppk.fill_inputs(2, 12, 8000, 4, nvband=2, seed=93)
+    inv = inp["inv_igp_index"].copy()
+    inv[5] = -1
+    inv[700] = 8000
+    q = dict(inp, inv_igp_index=inv)
+    a = handle.sigma(q, precision=FACT, acht_n1_loc=True)
+    dev = {k: (torch.from_numpy(np.ascontiguousarray(v)).cuda() if isinstance(v, np.ndarray) else v)
+           for k, v in q.items()}
+    torch.cuda.synchronize()
+    b = handle.sigma(dev, precision=FACT, memory=gppk.MEM_DEVICE, acht_n1_loc=True)
+    for k in ("achtemp", "asxtemp", "acht_n1_loc"):
+        assert np.array_equal(a[k], b[k]), k
+    ref = orc.sigma(q, want_acht_n1=True)
+    assert rel_err(a["acht_n1_loc"], ref["acht_n1_loc"]) <= FP64_TOL
+
+
+def test_factored_acht_n1_loc_canonical(handle):
+    """512/2/32768/20: the factored per-band log against the per-term FP64
+    kernel's, band by band."""
+    say("factored acht_n1_loc canonical")
+    inp = gppk.fill_inputs(2, 512, 32768, 20, nvband=2, seed=2024)
+    f = handle.sigma(inp, precision=FACT, acht_n1_loc=True)
+    d = handle.sigma(inp, acht_n1_loc=True)
+    assert rel_err(f["acht_n1_loc"], d["acht_n1_loc"]) <= 1e-12
+    say(f"with the per-band log: factored {f['unocc_kernel_ms']:.3f} ms, "
+        f"per-term {d['unocc_kernel_ms']:.3f} ms")
