@@ -26,8 +26,9 @@ The JSON line also carries:
   factored      the band-sum-first FP64 path (precision fp64_factored, only
                 where wx is shared by all bands): P = sum_n1 conj(aqsm) aqsn
                 on the FP64 matrix cores, then sum P * delw * I_eps; its
-                speed, its error against the oracle and against the per-term
-                FP64 result, and its own roofline (8 flops per non-occupied
+                speed (also with the per-band log acht_n1_loc), its error
+                against the oracle and against the per-term FP64 result, and
+                its own roofline (8 flops per non-occupied
                 (n1, my_igp, ig) against the FP64 matrix peak).  Reported
                 beside `value`, which stays a per-term path.
 
@@ -362,6 +363,18 @@ def run_single(args, name):
         f = runs[FACT]
         fl = FLOP_GEMM * (nb - nv) * ng * nc
         fa = fl / (f["unocc_kernel_ms"] * 1e-3) / 1e12
+        # the same evaluation with the per-band log (a second matrix product)
+        import torch
+        h.sigma(dev, precision=FACT, memory=gppk.MEM_DEVICE,
+                stream=torch.cuda.current_stream().cuda_stream, acht_n1_loc=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            h.sigma(dev, precision=FACT, memory=gppk.MEM_DEVICE,
+                    stream=torch.cuda.current_stream().cuda_stream, acht_n1_loc=True)
+        torch.cuda.synchronize()
+        fact_log_ms = (time.perf_counter() - t0) / 3 * 1e3
+        log(f"{FACT} with acht_n1_loc: {fact_log_ms:.3f} ms/step")
         ftraffic = None
         tf = ROOT / "profiles" / f"traffic_unocc_{FACT}_{nb}_{nv}_{nc}_{npg}.json"
         if tf.exists():
@@ -372,6 +385,7 @@ def run_single(args, name):
                                 if "fp64" in runs else None),
             "max_rel_err_vs_oracle": errs.get(FACT),
             "max_rel_err_vs_fp64_full": errs.get("factored_vs_fp64_full"),
+            "ms_with_acht_n1_loc": fact_log_ms,
             "roofline": {"bound": "mfma", "achieved": fa,
                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                          "frac": fa / PEAK_FP64_TFLOPS, "traffic": ftraffic,
