@@ -8,7 +8,10 @@
 // gppKer.cpp:151-201,237-239,376-381; the band loop (:270-361) is replaced
 // by one gpp_sigma_ex() call.  Inputs default to the file's constant fills
 // (config 1, :241-264).  asxtemp, which the reference accumulates but never
-// prints, is printed after achtemp.
+// prints, is printed after achtemp.  Without --precision the sum is taken
+// band sum first (factored) when all bands share the frequency grid (the
+// file's fills, config 1 and 2), as the drop-in gpp_sigma() does, and per
+// term (fp64) for band-resolved frequencies (config 3).
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -38,6 +41,7 @@ int main(int argc, char **argv) {
 
   gpp_options opt;
   gpp_default_options(&opt);
+  bool precision_given = false;
   int config = 1;
   unsigned long long seed = 12345;
   for (int i = 5; i < argc; ++i) {
@@ -47,6 +51,7 @@ int main(int argc, char **argv) {
       return a.compare(0, n, k) == 0 ? a.c_str() + n : nullptr;
     };
     if (const char *v = val("--precision=")) {
+      precision_given = true;
       opt.precision = std::strcmp(v, "mixed") == 0      ? GPP_PREC_MIXED
                       : std::strcmp(v, "factored") == 0 ? GPP_PREC_FP64_FACTORED
                                                         : GPP_PREC_FP64;
@@ -69,6 +74,8 @@ int main(int argc, char **argv) {
               << std::endl;
     return 2;
   }
+  if (!precision_given)
+    opt.precision = config == 3 ? GPP_PREC_FP64 : GPP_PREC_FP64_FACTORED;
   const int npes = 1;
   const int ngpown = ncouls / (nodes_per_group * npes);  // gppKer.cpp:163
   const int nstart = 0, nend = 3;

@@ -461,7 +461,7 @@ def test_shape_sweep_vs_oracle(handle, shape, config):
         check(handle.sigma(inp, semantics=sem), orc.sigma(inp, sem))
 
 
-@pytest.mark.parametrize("precision", ["fp64", "factored"])
+@pytest.mark.parametrize("precision", [None, "fp64", "factored"])
 def test_cli_matches_reference_binary_lines(precision):
     """`gpp nb nv nc npg` (the mini-app's constant fills) prints the achtemp
     lines of the reference binary (tests/golden/ref_constfill.json, printed
@@ -474,7 +474,8 @@ def test_cli_matches_reference_binary_lines(precision):
     for r in golden:
         args = [str(r[k]) for k in ("number_bands", "nvband", "ncouls", "nodes_per_group")]
         say("cli", args, precision)
-        out = subprocess.run([str(gppk.CLI_PATH)] + args + [f"--precision={precision}"],
+        extra = [f"--precision={precision}"] if precision else []
+        out = subprocess.run([str(gppk.CLI_PATH)] + args + extra,
                              capture_output=True, text=True, timeout=120, check=True).stdout
         got = {}
         for m in re.finditer(r"^achtemp\[(\d)\] = \(([^,]+),([^)]+)\)$", out, re.M):
