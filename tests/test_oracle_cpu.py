@@ -145,7 +145,10 @@ def test_oracle_closed_form(shape, sem):
         assert rel_err(got["asxtemp"], asx) < 1e-14
 
 
-def test_oracle_closed_form_values_from_baseline():
+def test_oracle_closed_form_values_64_2_2048_8():
+    """The mini-app fills at 64/2/2048/8: achtemp[iw] = NB (NC/4) c delw(wx)
+    sum_igp vcoul and asxtemp[iw] = NV NG NC c^3 / (2 wx^2), c = 0.5+0.5i,
+    wx = 3, 4, 5, evaluated by hand."""
     inp = gppk.fill_inputs(1, 64, 2048, 8, nvband=2)
     got = orc.sigma(inp)
     want_ach = [-322638.769230769 + 1613193.84615385j, -167772.16 + 1174405.12j,
