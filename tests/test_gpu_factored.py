@@ -200,3 +200,19 @@ def test_factored_acht_n1_loc_canonical(handle):
     assert rel_err(f["acht_n1_loc"], d["acht_n1_loc"]) <= 1e-12
     say(f"with the per-band log: factored {f['unocc_kernel_ms']:.3f} ms, "
         f"per-term {d['unocc_kernel_ms']:.3f} ms")
+
+
+@pytest.mark.parametrize("shape", [(1, 0, 1, 1), (2, 1, 1, 1), (1, 1, 5, 5), (3, 0, 2, 2),
+                                   (6, 2, 65, 65), (4, 1, 63, 1), (65, 1, 64, 1)])
+def test_factored_tiny_and_boundary_shapes(handle, shape):
+    """One G-vector, one G' row, all bands occupied, tile boundaries (63, 64,
+    65 G-vectors or G' rows; 64 non-occupied bands = one band tile) in every
+    semantics, with the per-band log."""
+    nb, nv, nc, npg = shape
+    say("factored tiny", shape)
+    inp = gppk.fill_inputs(2, nb, nc, npg, nvband=nv, seed=7 * nb + nc)
+    for sem in ("gppker", "strict", "bgw"):
+        got = handle.sigma(inp, precision=FACT, semantics=sem, acht_n1_loc=True)
+        ref = orc.sigma(inp, sem, want_acht_n1=True)
+        check(got, ref)
+        assert rel_err(got["acht_n1_loc"], ref["acht_n1_loc"]) <= FP64_TOL
