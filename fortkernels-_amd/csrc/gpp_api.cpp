@@ -726,9 +726,19 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   if (o->flags & (GPP_FLAG_ALL_BANDS_UNOCC | GPP_FLAG_SKIP_UNOCC))
     return set_err(h, GPP_EUNSUPPORTED,
                    "band-range flags are not supported in batches");
-  if (o->precision == GPP_PREC_FP64_FACTORED)
-    return set_err(h, GPP_EUNSUPPORTED,
-                   "the factored band sum is not available in batches");
+  if (o->precision == GPP_PREC_FP64_FACTORED) {
+    // Band sum first: each problem's matrix-core launch is already a full
+    // grid; the problems run one after another on the same stream.
+    gpp_options one = *o;
+    one.flags &= ~GPP_FLAG_AUTOTUNE;
+    for (int i = 0; i < n; ++i) {
+      gpp_c128 *keep = r[i].acht_n1_loc;
+      std::memset(r + i, 0, sizeof(gpp_result));
+      r[i].acht_n1_loc = keep;  // NULL (checked above)
+      if ((rc = gpp_sigma_ex(h, p + i, &one, r + i))) return rc;
+    }
+    return GPP_OK;
+  }
   HIP_OR_RET(h, hipSetDevice(h->device));
   hipStream_t s = (o->memory == GPP_MEM_DEVICE && o->stream)
                       ? (hipStream_t)o->stream

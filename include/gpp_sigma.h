@@ -61,8 +61,9 @@ typedef struct { double re, im; } gpp_c128;
                                 products are likewise summed before their
                                 solver; acht_n1_loc by a second product
                                 over the G-vectors.  Needs wx shared by
-                                all bands (wx_per_band = 0); no batches
-                                or tile tuning                            */
+                                all bands (wx_per_band = 0); batches run
+                                their problems one after another; no
+                                tile tuning                               */
 
 /* Which variant of the reference physics to follow. */
 #define GPP_SEM_GPPKER         0 /* gppKer.cpp: unmasked non-occupied sch;
