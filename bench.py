@@ -339,7 +339,7 @@ def run_single(args, name):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "mixed(f32 terms, f64 sums)",
+        "dtype": "f64" if best in ("fp64", FACT) else "mixed(f32 terms, f64 sums)",
         "data": "synthetic",
         "config": {"workload": w["desc"], "number_bands": nb, "nvband": nv,
                    "ncouls": nc, "nodes_per_group": npg, "ngpown": ng,
@@ -637,7 +637,7 @@ def run_batch(args, name="batch"):
         "n_gpus": 1, "steps": min(args.steps, 5), "warmup": max(1, args.warmup),
         "ms_per_step": out[best]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "mixed(f32 terms, f64 sums)",
+        "dtype": "f64" if best in ("fp64", FACT) else "mixed(f32 terms, f64 sums)",
         "data": "synthetic",
         "config": {"workload": w["desc"], "instances": w["n"], "terms": terms,
                    "resident_GB": gbytes, "path": best},
