@@ -775,7 +775,17 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     put(&n, sizeof n);
     put(o, sizeof *o);
     put(&tile, sizeof tile);
-    put(p, sizeof(gpp_problem) * n);
+    // Field by field: a caller's structs may carry uninitialised padding.
+    for (int i = 0; i < n; ++i) {
+      const gpp_problem &q = p[i];
+      const int ints[5] = {q.number_bands, q.nvband, q.ncouls, q.ngpown,
+                           q.wx_per_band};
+      const void *ptrs[8] = {q.aqsmtemp, q.aqsntemp, q.I_eps_array,
+                             q.wtilde_array, q.vcoul, q.inv_igp_index,
+                             q.indinv, q.wx_array};
+      put(ints, sizeof ints);
+      put(ptrs, sizeof ptrs);
+    }
   }
   void *bases_now[4] = {h->part.ptr, h->meta.ptr, h->work.ptr, (void *)s};
   if (!key.empty() && h->graph_exec && h->batch_key == key &&
