@@ -164,8 +164,11 @@ GPP_API int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *
  * launch per band class: the blocks of all problems form one work list,
  * sorted largest-first, run by a single grid; the sequence is captured in a
  * hipGraph that is replayed while the batch (pointers, shapes, options) is
- * unchanged.  All problems use the same options and memory space;
- * band-range flags and acht_n1_loc are not available here. */
+ * unchanged (a repeated device-memory call replays it without planning
+ * again; the arrays' contents may change between calls).  All problems use
+ * the same options and memory space; band-range flags and acht_n1_loc are
+ * not available here.  GPP_PREC_FP64_FACTORED runs the problems one after
+ * another. */
 GPP_API int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
                     const gpp_options *o, gpp_result *r);
 
