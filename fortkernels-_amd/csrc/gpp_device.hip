@@ -21,6 +21,10 @@
 //     order.  No atomics touch the sums, so results are bit-reproducible.
 // The occupied bands (n1 < nvband, gppKer.cpp:122-146 / 16-62) run in a
 // separate FP64 kernel with one block per (my_igp, ig chunk).
+// When every band shares the frequency grid, the non-occupied sum can also
+// be taken band sum first: a complex product over the bands on the FP64
+// matrix cores, contracted with delw * I_eps (unocc_gemm_kernel; its
+// per-band log by a second product over the G-vectors, n1log_gemm_kernel).
 #include "gpp_internal.h"
 
 namespace gpp {
