@@ -233,3 +233,30 @@ def test_factored_batch_runs_each_problem(handle):
         handle.batch([inputs[0], gppk.fill_inputs(3, 4, 200, 4, nvband=1, seed=3)],
                      precision=FACT)
     assert ei.value.code == gppk.GPP_EUNSUPPORTED
+
+
+def _wide_shapes(n, seed):
+    """Random shapes with more G-vectors and G' rows than the ragged sweep
+    (several 2048-wide chunks and 64-row groups), nvband up to
+    number_bands."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nb = int(rng.integers(1, 48))
+        nv = int(rng.integers(0, min(nb, 6) + 1))
+        npg = int(rng.integers(6, 16))
+        nc = int(rng.integers(64, 4500))
+        out.append((nb, nv, nc, npg))
+    return out
+
+
+@pytest.mark.parametrize("shape", _wide_shapes(12, 77))
+def test_factored_wide_sweep_with_log(handle, shape):
+    nb, nv, nc, npg = shape
+    say("factored wide sweep", shape)
+    inp = gppk.fill_inputs(2, nb, nc, npg, nvband=nv, seed=3 * nb + nc)
+    for sem in ("gppker", "bgw"):
+        got = handle.sigma(inp, precision=FACT, semantics=sem, acht_n1_loc=True)
+        ref = orc.sigma(inp, sem, want_acht_n1=True)
+        check(got, ref)
+        assert rel_err(got["acht_n1_loc"], ref["acht_n1_loc"]) <= FP64_TOL
