@@ -232,7 +232,9 @@ __device__ __forceinline__ double clamp_ratio(double x) {
 //   4    Im brd      5  right = zb     6  left = zb[f-1]     7  (Im brd)^2
 //   8,9  i pref_zb avg (sch2 factor)   10,11 schDt[f-1]
 //   12,13 lin/(right-left)             14,15 lin = schDt[f] - schDt[f-1]
-__global__ void freq_prep_kernel(FreqArgs a, double *__restrict__ rows) {
+// (ffKernel.cpp semantics: 3 = r1 r4 and 5 = r0 r4 instead.)
+__global__ void freq_prep_kernel(FreqArgs a, int sem,
+                                 double *__restrict__ rows) {
   const size_t n = (size_t)a.NB * a.nFreq;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -269,6 +271,13 @@ __global__ void freq_prep_kernel(FreqArgs a, double *__restrict__ rows) {
       r[6] = 0.0;
 #pragma unroll
       for (int k = 8; k < kRowDoubles; ++k) r[k] = 0.0;
+    }
+    if (sem == 0) {
+      // ffKernel.cpp reads neither the cor shift nor the interval ends; its
+      // pole term uses the frequency-independent products i pref schDt *
+      // Im brd instead (rows 3 and 5 then hold r1 r4 and r0 r4).
+      r[3] = r[1] * r[4];
+      r[5] = r[0] * r[4];
     }
     double2 *o = (double2 *)(rows + i * kRowDoubles);
 #pragma unroll
@@ -317,10 +326,10 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
       const double xa = wx - r[2], xb = wx - rn[2];
       double qa, qb;
       recip2(fma(xa, xa, r[7]), fma(xb, xb, rn[7]), qa, qb);
-      achr -= (r[0] * xa + r[1] * r[4]) * qa;
-      achi -= (r[1] * xa - r[0] * r[4]) * qa;
-      achr -= (rn[0] * xb + rn[1] * rn[4]) * qb;
-      achi -= (rn[1] * xb - rn[0] * rn[4]) * qb;
+      achr -= fma(r[0], xa, r[3]) * qa;
+      achi -= fma(r[1], xa, -r[5]) * qa;
+      achr -= fma(rn[0], xb, rn[3]) * qb;
+      achi -= fma(rn[1], xb, -rn[5]) * qb;
     }
   }
   for (; f < f1; ++f) {
@@ -455,7 +464,7 @@ hipError_t launch_freq(const FreqArgs &a, int sem, int splits, double *rows,
   const size_t n = (size_t)a.NB * a.nFreq;
   size_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  freq_prep_kernel<<<(unsigned)blocks, 256, 0, s>>>(a, rows);
+  freq_prep_kernel<<<(unsigned)blocks, 256, 0, s>>>(a, sem, rows);
   const int fchunk = (a.nFreq + splits - 1) / splits;
   const int nocc = std::min(a.NV, a.NB), nun = a.NB - nocc;
   const unsigned gx = (unsigned)((a.NE + 255) / 256);
