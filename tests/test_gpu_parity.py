@@ -485,3 +485,24 @@ def test_cli_matches_reference_binary_lines(precision):
         assert rel_err(g, ref) < 5e-12, (args, g, ref)
         ach, _ = closed_form(int(args[0]), int(args[1]), int(args[2]), int(args[3]), "gppker")
         assert rel_err(g, ach) < 1e-13, (args, g, ach)
+
+
+def test_batch_replay_switches_between_batches(handle):
+    """Alternating two device-memory batches (and two tiles) on one handle:
+    every call must give its own batch's results, whether it replays the
+    captured graph or plans again."""
+    import torch
+    say("batch replay switching")
+    a = [gppk.fill_inputs_device(3, nb, nc, 8, nvband=nv, seed=500 + i)
+         for i, (nb, nc, nv) in enumerate([(4, 300, 1), (7, 520, 2)])]
+    b = [gppk.fill_inputs_device(2, nb, nc, 8, nvband=nv, seed=600 + i)
+         for i, (nb, nc, nv) in enumerate([(5, 800, 2), (3, 260, 0), (6, 400, 3)])]
+    torch.cuda.synchronize()
+    host = lambda d: {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in d.items()}
+    refs = {id(d): orc.sigma(host(d)) for d in a + b}
+    ta, tb = gppk.BatchProblems(a), gppk.BatchProblems(b)
+    for batch, table, tile in ((a, ta, 0), (b, tb, 0), (a, ta, 0), (a, ta, 4), (b, tb, 0), (b, tb, 0)):
+        res = handle.batch(table, memory=gppk.MEM_DEVICE, tile_ig=tile)
+        assert len(res) == len(batch)
+        for d, r in zip(batch, res):
+            check(r, refs[id(d)])
