@@ -149,10 +149,27 @@ FF_ARRAYS = ("aqsmtemp", "aqsntemp", "I_epsR_array", "I_epsA_array", "vcoul",
 def make_ff_problem(inp: dict) -> FFProblem:
     """FFProblem from a dict of arrays (numpy host or torch device) with the
     keys of FF_ARRAYS plus number_bands, nvband, ncouls, ngpown, nFreq,
-    nfreqeval, freqevalmin, freqevalstep, pref_zb, occ."""
-    return FFProblem(int(inp["number_bands"]), int(inp["nvband"]),
-                     int(inp["ncouls"]), int(inp["ngpown"]), int(inp["nFreq"]),
-                     int(inp["nfreqeval"]), *[_ptr(inp.get(k)) for k in FF_ARRAYS],
+    nfreqeval, freqevalmin, freqevalstep, pref_zb, occ.  Arrays are checked
+    as in make_problem (dtype, element count, one memory space); absent
+    optional ones pass NULL."""
+    nb, nc, ng = int(inp["number_bands"]), int(inp["ncouls"]), int(inp["ngpown"])
+    nf = int(inp["nFreq"])
+    ptrs = _checked(inp, (
+        ("aqsmtemp", "c128", nb * nc, True),
+        ("aqsntemp", "c128", nb * nc, True),
+        ("I_epsR_array", "c128", nf * ng * nc, False),
+        ("I_epsA_array", "c128", nf * ng * nc, False),
+        ("vcoul", "f64", nc, True),
+        ("inv_igp_index", "i32", ng, True),
+        ("indinv", "i32", nc, True),
+        ("ekq", "f64", nb, False),
+        ("dFreqGrid", "f64", nf, False),
+        ("pref", "f64", nf, False),
+        ("dFreqBrd", "c128", nf, False),
+        ("schDt_matrix", "c128", nb * nf, False),
+    ), "ff problem")
+    return FFProblem(nb, int(inp["nvband"]), nc, ng, nf,
+                     int(inp["nfreqeval"]), *ptrs,
                      float(inp["freqevalmin"]), float(inp["freqevalstep"]),
                      float(inp["pref_zb"]), float(inp["occ"]))
 
@@ -232,24 +249,88 @@ def _ptr(a) -> int | None:
     raise TypeError(f"unsupported array type {type(a)}")
 
 
+# Element kinds the C ABI reads: complex128 (or float64 with a trailing
+# re/im axis of 2), float64, int32.
+_KIND_DTYPES = {
+    "c128": ("complex128",),
+    "f64": ("float64",),
+    "i32": ("int32",),
+}
+
+
+def _space(a) -> str:
+    """'host' for numpy arrays and CPU tensors, 'device' for GPU tensors."""
+    if isinstance(a, np.ndarray):
+        return "host"
+    if hasattr(a, "data_ptr"):
+        return "device" if getattr(a, "is_cuda", False) else "host"
+    raise TypeError(f"unsupported array type {type(a)}")
+
+
+def _elements(a, kind: str, name: str) -> int:
+    """Element count of `a` as the C side reads it; raises ValueError when
+    its dtype is not the one the C ABI expects for `kind`."""
+    dt = str(a.dtype).replace("torch.", "")
+    size = int(a.size) if isinstance(a, np.ndarray) else int(a.numel())
+    if dt in _KIND_DTYPES[kind]:
+        return size
+    if kind == "c128" and dt == "float64" and a.ndim >= 1 and a.shape[-1] == 2:
+        return size // 2
+    raise ValueError(f"{name}: dtype {dt}, expected "
+                     f"{'complex128' if kind == 'c128' else _KIND_DTYPES[kind][0]}")
+
+
+def _checked(inp: dict, spec, what: str) -> list:
+    """Checks every array of `spec` ((key, kind, minimum element count,
+    required) tuples) against its dtype and size and that all of them live
+    in one memory space; returns their pointers in order (None for absent
+    optional arrays)."""
+    spaces = set()
+    ptrs = []
+    for key, kind, need, required in spec:
+        a = inp.get(key)
+        if a is None:
+            if required and need > 0:
+                raise ValueError(f"{what}: {key} is missing")
+            ptrs.append(None)
+            continue
+        n = _elements(a, kind, key)
+        if n < need:
+            raise ValueError(f"{what}: {key} holds {n} elements, the shape "
+                             f"needs {need}")
+        spaces.add(_space(a))
+        ptrs.append(_ptr(a))
+    if len(spaces) > 1:
+        raise ValueError(f"{what}: arrays mix host and device memory")
+    return ptrs
+
+
 def make_problem(inp: dict) -> Problem:
     """Problem struct from a dict of arrays (numpy host or torch device).
     Keys: aqsmtemp, aqsntemp, I_eps_array, wtilde_array (complex128, or
     float64 with a trailing re/im axis), vcoul (float64), inv_igp_index,
     indinv (int32), wx_array (float64, [3] or [NB,3]), nvband, and
-    optionally number_bands/ncouls/ngpown."""
+    optionally number_bands/ncouls/ngpown.  Every array is checked against
+    the dtype and element count the shape needs (ValueError otherwise), and
+    all must be in one memory space: the C side reads them as raw pointers."""
     aqsm = inp["aqsmtemp"]
     ieps = inp["I_eps_array"]
-    nb = inp.get("number_bands", aqsm.shape[0])
-    nc = inp.get("ncouls", aqsm.shape[1] if aqsm.ndim >= 2 else inp["vcoul"].shape[0])
-    ng = inp.get("ngpown", ieps.shape[0])
+    nb = int(inp.get("number_bands", aqsm.shape[0]))
+    nc = int(inp.get("ncouls", aqsm.shape[1] if aqsm.ndim >= 2 else inp["vcoul"].shape[0]))
+    ng = int(inp.get("ngpown", ieps.shape[0]))
     wx = inp["wx_array"]
     per_band = int(wx.ndim == 2 or wx.shape[0] != 3)
-    return Problem(nb, int(inp["nvband"]), nc, ng,
-                   _ptr(aqsm), _ptr(inp["aqsntemp"]), _ptr(ieps),
-                   _ptr(inp["wtilde_array"]), _ptr(inp["vcoul"]),
-                   _ptr(inp["inv_igp_index"]), _ptr(inp["indinv"]),
-                   _ptr(wx), per_band)
+    ptrs = _checked(inp, (
+        ("aqsmtemp", "c128", nb * nc, True),
+        ("aqsntemp", "c128", nb * nc, True),
+        ("I_eps_array", "c128", ng * nc, True),
+        ("wtilde_array", "c128", ng * nc, True),
+        ("vcoul", "f64", nc, True),
+        ("inv_igp_index", "i32", ng, True),
+        ("indinv", "i32", nc, True),
+        ("wx_array", "f64", 3 * nb if per_band else 3, True),
+    ), "gpp problem")
+    return Problem(nb, int(inp["nvband"]), nc, ng, *ptrs, per_band)
 
 
 class BatchProblems:

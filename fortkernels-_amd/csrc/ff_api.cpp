@@ -628,13 +628,12 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
 namespace {
 
 int run_cpp(const ff_problem &p, int parts, ff_result *r) {
-  int rc;
-  gpp_handle h = gpp::default_handle(&rc);
-  if (rc) return rc;
+  gpp::DefaultHandle dh;
+  if (dh.rc) return dh.rc;
   ff_options o;
   ff_default_options(&o);
   o.parts = parts;
-  return ff_sigma_ex(h, &p, &o, r);
+  return dh.ret(ff_sigma_ex(dh.h, &p, &o, r));
 }
 
 void add_into(gpp_c128 *dst, const std::vector<gpp_c128> &v) {

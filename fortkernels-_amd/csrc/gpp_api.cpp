@@ -53,15 +53,17 @@ constexpr double kDefaultResonance = 1e-6;
 
 namespace gpp {
 
+// Error text of the calling thread's last failed handle-less call (the
+// drop-in entry points and gpp_sigma_noalloc), for gpp_last_error(NULL).
+thread_local std::string g_thread_err;
+
 int set_err(gpp_handle h, int code, const char *fmt, ...) {
-  if (h) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    h->err = buf;
-  }
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  (h ? h->err : g_thread_err) = buf;
   return code;
 }
 
@@ -110,18 +112,37 @@ Stager *stager(gpp_handle h, int *rc) {
   return h->stager;
 }
 
-gpp_handle default_handle(int *rc) {
-  static std::mutex mu;
-  static gpp_handle h = nullptr;
-  std::lock_guard<std::mutex> lock(mu);
-  *rc = GPP_OK;
-  if (!h) *rc = gpp_create(&h, -1);
-  return h;
+namespace {
+std::mutex g_default_mu;                  // held for a whole drop-in call
+constexpr int kMaxDefaultDevices = 64;
+gpp_handle g_default[kMaxDefaultDevices];  // one per device, never destroyed
+}  // namespace
+
+DefaultHandle::DefaultHandle() {
+  g_default_mu.lock();
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (dev < 0 || dev >= kMaxDefaultDevices) {
+    rc = set_err(nullptr, GPP_EINVAL, "device %d beyond the %d default "
+                 "handles", dev, kMaxDefaultDevices);
+    return;
+  }
+  if (!g_default[dev]) {
+    rc = gpp_create(&g_default[dev], dev);
+    if (rc) {
+      set_err(nullptr, rc, "gpp_create on device %d failed (%d)", dev, rc);
+      return;
+    }
+  }
+  h = g_default[dev];
 }
 
-}  // namespace gpp
+DefaultHandle::~DefaultHandle() { g_default_mu.unlock(); }
 
-namespace {
+int DefaultHandle::ret(int code) {
+  if (code && h) g_thread_err = h->err;
+  return code;
+}
 
 int validate(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
   if (!p || !o) return set_err(h, GPP_EINVAL, "null problem or options");
@@ -167,6 +188,12 @@ int validate(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
     return set_err(h, GPP_EINVAL, "waves_per_block must be in [0,4]");
   return GPP_OK;
 }
+
+}  // namespace gpp
+
+namespace {
+
+using gpp::validate;
 
 int ilog2_ceil(long long x) {
   int l = 0;
@@ -491,7 +518,7 @@ int gpp_destroy(gpp_handle h) {
 }
 
 const char *gpp_last_error(gpp_handle h) {
-  if (!h) return "null handle";
+  if (!h) return gpp::g_thread_err.c_str();
   return h->err.c_str();
 }
 
@@ -975,9 +1002,8 @@ int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
               const int *indinv, const double *wx_array, gpp_c128 *achtemp,
               gpp_c128 *asxtemp) {
   if (!achtemp || !asxtemp) return GPP_EINVAL;
-  int rc;
-  gpp_handle h = gpp::default_handle(&rc);
-  if (rc) return rc;
+  gpp::DefaultHandle dh;
+  if (dh.rc) return dh.rc;
   gpp_problem p = {number_bands, nvband,   ncouls,        ngpown,
                    aqsmtemp,     aqsntemp, I_eps_array,   wtilde_array,
                    vcoul,        inv_igp_index, indinv,   wx_array,
@@ -988,7 +1014,7 @@ int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
   o.precision = GPP_PREC_FP64_FACTORED;
   gpp_result res;
   std::memset(&res, 0, sizeof res);
-  rc = gpp_sigma_ex(h, &p, &o, &res);
+  const int rc = dh.ret(gpp_sigma_ex(dh.h, &p, &o, &res));
   if (rc) return rc;
   for (int i = 0; i < 3; ++i) {
     achtemp[i] = res.achtemp[i];
@@ -1008,9 +1034,8 @@ int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
   if (!achtemp_re0 || !achtemp_re1 || !achtemp_re2 || !achtemp_im0 ||
       !achtemp_im1 || !achtemp_im2)
     return GPP_EINVAL;
-  int rc;
-  gpp_handle h = gpp::default_handle(&rc);
-  if (rc) return rc;
+  gpp::DefaultHandle dh;
+  if (dh.rc) return dh.rc;
   gpp_problem p = {number_bands, 0,        ncouls,        ngpown,
                    aqsmtemp,     aqsntemp, I_eps_array,   wtilde_array,
                    vcoul,        inv_igp_index, indinv,   wx_array,
@@ -1021,7 +1046,7 @@ int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
   o.flags = GPP_FLAG_ALL_BANDS_UNOCC | GPP_FLAG_NO_ACHSTEMP;
   gpp_result res;
   std::memset(&res, 0, sizeof res);
-  rc = gpp_sigma_ex(h, &p, &o, &res);
+  const int rc = dh.ret(gpp_sigma_ex(dh.h, &p, &o, &res));
   if (rc) return rc;
   // The reference adds into its reduction variables (:156-157,194-199).
   *achtemp_re0 += res.achtemp[0].re;
@@ -1041,9 +1066,8 @@ int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
                           int ngpown, const double *vcoul, int nstart,
                           int nend, double occ, int number_bands) {
   if (!asxtemp || nstart < 0 || nend > 3 || nstart > nend) return GPP_EINVAL;
-  int rc;
-  gpp_handle h = gpp::default_handle(&rc);
-  if (rc) return rc;
+  gpp::DefaultHandle dh;
+  if (dh.rc) return dh.rc;
   gpp_problem p = {number_bands, nvband,   ncouls,        ngpown,
                    aqsmtemp,     aqsntemp, I_eps_array,   wtilde_array,
                    vcoul,        inv_igp_index, indinv,   wx_array,
@@ -1055,7 +1079,7 @@ int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
   o.flags = GPP_FLAG_SKIP_UNOCC | GPP_FLAG_NO_ACHSTEMP;
   gpp_result res;
   std::memset(&res, 0, sizeof res);
-  rc = gpp_sigma_ex(h, &p, &o, &res);
+  const int rc = dh.ret(gpp_sigma_ex(dh.h, &p, &o, &res));
   if (rc) return rc;
   for (int iw = nstart; iw < nend; ++iw) {
     asxtemp[iw].re += res.asxtemp[iw].re * occ;

@@ -47,8 +47,27 @@ GPP_INTERNAL int ensure_host_out(gpp_handle h, size_t doubles);
 // The handle's pinned-ring stager, created and initialised on first use.
 GPP_INTERNAL Stager *stager(gpp_handle h, int *rc);
 // Library-owned handle on the current device for the reference-shaped
-// entry points (one per process, behind a mutex).
-GPP_INTERNAL gpp_handle default_handle(int *rc);
+// entry points: one per device, created on first use.  The object holds a
+// process-wide lock for its whole lifetime, so that concurrent drop-in
+// calls from several host threads run one after another on the shared
+// handle (its buffers, stream and error text).  On failure the handle's
+// error text is also kept as the calling thread's handle-less error
+// (gpp_last_error(NULL)).
+struct DefaultHandle {
+  DefaultHandle();
+  ~DefaultHandle();
+  DefaultHandle(const DefaultHandle &) = delete;
+  DefaultHandle &operator=(const DefaultHandle &) = delete;
+  int rc = 0;            // GPP_OK once h is usable
+  gpp_handle h = nullptr;
+  // Passes a drop-in call's return code through, recording the handle's
+  // error text for gpp_last_error(NULL) when it is nonzero.
+  int ret(int code);
+};
+
+// Validation shared by every entry point that takes a gpp_problem.
+GPP_INTERNAL int validate(gpp_handle h, const gpp_problem *p,
+                          const gpp_options *o);
 
 }  // namespace gpp
 

@@ -153,7 +153,10 @@ typedef struct gpp_handle_s *gpp_handle;
 /* Handle lifetime.  device < 0 selects the current HIP device. */
 GPP_API int gpp_create(gpp_handle *h, int device);
 GPP_API int gpp_destroy(gpp_handle h);
-GPP_API const char *gpp_last_error(gpp_handle h);   /* never NULL */
+/* Error text of the handle's last failed call; gpp_last_error(NULL) gives
+ * that of the calling thread's last failed call without a handle (the
+ * drop-in entry points).  Never NULL. */
+GPP_API const char *gpp_last_error(gpp_handle h);
 GPP_API void gpp_default_options(gpp_options *o);
 
 /* The general entry point. */
