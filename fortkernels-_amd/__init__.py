@@ -30,6 +30,7 @@ GPP_ENOMEM = -2
 GPP_EDEVICE = -3
 GPP_ENONFINITE = -4
 GPP_EUNSUPPORTED = -5
+GPP_ESCRATCH = -6
 
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -64,7 +65,7 @@ EXPORTS = (
     "gpp_create", "gpp_destroy", "gpp_last_error", "gpp_default_options",
     "gpp_sigma_ex", "gpp_sigma_batch", "gpp_autotune", "gpp_sigma",
     "gpp_achtemp_kernel", "gpp_tillNvband_kernel", "gpp_fill_inputs",
-    "gpp_fill_inputs_device", "gpp_version",
+    "gpp_fill_inputs_device", "gpp_sigma_noalloc", "gpp_version",
     "ff_default_options", "ff_sigma_ex", "ff_achsDtemp_Kernel",
     "ff_asxDtemp_Kernel", "ff_achDtemp_Kernel", "ff_achDtemp_cor_Kernel",
     "ff_fill_inputs", "ff_fill_inputs_device",
@@ -212,6 +213,7 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     lib.gpp_fill_inputs.argtypes = [ip, C.c_ulonglong, ip, ip, ip, ip] + [vp] * 8
     lib.gpp_fill_inputs_device.argtypes = ([ip, C.c_ulonglong, ip, ip, ip, ip]
                                            + [vp] * 9)
+    lib.gpp_sigma_noalloc.argtypes = [C.POINTER(Problem), C.POINTER(Options), vp]
     lib.gpp_version.restype = C.c_char_p
     lib.ff_default_options.argtypes = [C.POINTER(FFOptions)]
     lib.ff_sigma_ex.argtypes = [vp, C.POINTER(FFProblem), C.POINTER(FFOptions),

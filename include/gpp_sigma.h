@@ -48,6 +48,8 @@ typedef struct { double re, im; } gpp_c128;
 #define GPP_ENONFINITE  -4   /* a result is NaN/Inf and the caller asked
                                 for GPP_FLAG_CHECK_FINITE                  */
 #define GPP_EUNSUPPORTED -5  /* option combination not implemented         */
+#define GPP_ESCRATCH    -6   /* gpp_sigma_noalloc: the problem needs more
+                                scratch than GPP_NOALLOC_SCRATCH_BYTES      */
 
 /* Arithmetic of the non-occupied (n1 >= nvband) sum. */
 #define GPP_PREC_FP64    0   /* complex128 throughout                      */
@@ -155,7 +157,7 @@ GPP_API int gpp_create(gpp_handle *h, int device);
 GPP_API int gpp_destroy(gpp_handle h);
 /* Error text of the handle's last failed call; gpp_last_error(NULL) gives
  * that of the calling thread's last failed call without a handle (the
- * drop-in entry points).  Never NULL. */
+ * drop-in entry points, gpp_sigma_noalloc).  Never NULL. */
 GPP_API const char *gpp_last_error(gpp_handle h);
 GPP_API void gpp_default_options(gpp_options *o);
 
@@ -251,6 +253,31 @@ GPP_API int gpp_fill_inputs_device(int config, unsigned long long seed,
                            gpp_c128 *wtilde_array, double *vcoul,
                            int *inv_igp_index, int *indinv, double *wx_array,
                            void *stream);
+
+/*
+ * Allocation-free evaluation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory during the call (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It is gpp_sigma_ex for device-resident inputs (o->memory must be
+ * GPP_MEM_DEVICE) with FP64 or GPP_PREC_FP64_FACTORED arithmetic: the same
+ * kernels and record layout, so the same bits for the same tile (tile_ig /
+ * waves_per_block 0 select K=8 x 4 waves, the untuned default).  Its scratch
+ * is a library-owned device array of GPP_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends; a problem that needs more (about
+ * 16*number_bands*ngpown bytes plus 96 per block record; the canonical
+ * 512/2/32768/20 shape takes 19 MB) is refused with GPP_ESCRATCH before
+ * anything is launched.  The call enqueues its kernels and one
+ * device-to-device copy on o->stream (NULL: the null stream) and returns
+ * without waiting; out is a device buffer of 7 gpp_c128: achtemp[3],
+ * asxtemp[3], achstemp.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (two streams, two host threads) race and
+ * must be serialised by the caller.  Not supported: mixed precision,
+ * acht_n1_loc, GPP_FLAG_AUTOTUNE and GPP_FLAG_CHECK_FINITE (EUNSUPPORTED).
+ * Errors: gpp_last_error(NULL).
+ */
+#define GPP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+GPP_API int gpp_sigma_noalloc(const gpp_problem *p, const gpp_options *o,
+                              gpp_c128 *out);
 
 /* Version string of the library build. */
 GPP_API const char *gpp_version(void);
