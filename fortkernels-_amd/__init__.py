@@ -519,13 +519,18 @@ class Handle:
         return k.value, w.value
 
 
+FILL_VCOUL_LINEAR = 256
+
+
 def fill_inputs(config: int, number_bands: int, ncouls: int,
                 nodes_per_group: int, nvband: int, seed: int = 12345,
-                lib: C.CDLL | None = None) -> dict:
+                lib: C.CDLL | None = None, vcoul_linear: bool = False) -> dict:
     """Synthetic inputs (numpy, host) from the library's generator
     (gpp_fill_inputs): config 1 constant fills, 2 canonical random,
-    3 band-resolved energies with near-resonant wtilde."""
+    3 band-resolved energies with near-resonant wtilde; vcoul_linear
+    replaces vcoul by vcoul[i] = i / ncouls."""
     lib = lib or load_library()
+    flag = FILL_VCOUL_LINEAR if vcoul_linear else 0
     nb, nc = number_bands, ncouls
     ng = nc // nodes_per_group
     d = {
@@ -538,7 +543,7 @@ def fill_inputs(config: int, number_bands: int, ncouls: int,
         "indinv": np.empty(nc, np.int32),
         "wx_array": np.empty((nb, 3) if config == 3 else 3, np.float64),
     }
-    rc = lib.gpp_fill_inputs(config, seed, nb, nc, ng, nodes_per_group,
+    rc = lib.gpp_fill_inputs(config | flag, seed, nb, nc, ng, nodes_per_group,
                              *[_ptr(d[k]) for k in (
                                  "aqsmtemp", "aqsntemp", "I_eps_array",
                                  "wtilde_array", "vcoul", "inv_igp_index",
@@ -552,11 +557,13 @@ def fill_inputs(config: int, number_bands: int, ncouls: int,
 
 def fill_inputs_device(config: int, number_bands: int, ncouls: int,
                        nodes_per_group: int, nvband: int, seed: int = 12345,
-                       lib: C.CDLL | None = None) -> dict:
+                       lib: C.CDLL | None = None,
+                       vcoul_linear: bool = False) -> dict:
     """The same synthetic inputs generated in device memory (torch tensors on
     cuda:0, complex128 / float64 / int32), by gpp_fill_inputs_device."""
     import torch
     lib = lib or load_library()
+    flag = FILL_VCOUL_LINEAR if vcoul_linear else 0
     nb, nc = number_bands, ncouls
     ng = nc // nodes_per_group
     dev = torch.device("cuda", 0)
@@ -572,7 +579,7 @@ def fill_inputs_device(config: int, number_bands: int, ncouls: int,
                                 dtype=torch.float64, device=dev),
     }
     stream = torch.cuda.current_stream().cuda_stream
-    rc = lib.gpp_fill_inputs_device(config, seed, nb, nc, ng, nodes_per_group,
+    rc = lib.gpp_fill_inputs_device(config | flag, seed, nb, nc, ng, nodes_per_group,
                                     *[d[k].data_ptr() for k in (
                                         "aqsmtemp", "aqsntemp", "I_eps_array",
                                         "wtilde_array", "vcoul",

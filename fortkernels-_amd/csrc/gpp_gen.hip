@@ -84,6 +84,8 @@ extern "C" int gpp_fill_inputs_device(
     gpp_c128 *I_eps_array, gpp_c128 *wtilde_array, double *vcoul,
     int *inv_igp_index, int *indinv, double *wx_array, void *stream) {
   const int NB = number_bands, NC = ncouls, NG = ngpown;
+  const bool vlin = config & GPP_FILL_VCOUL_LINEAR;
+  config &= ~GPP_FILL_VCOUL_LINEAR;
   if (config < 1 || config > 3 || NB < 0 || NC <= 0 || NG < 0)
     return GPP_EINVAL;
   if (!aqsmtemp || !aqsntemp || !I_eps_array || !wtilde_array || !vcoul ||
@@ -125,6 +127,8 @@ extern "C" int gpp_fill_inputs_device(
         }
     }
   }
+  if (vlin)
+    for (int i = 0; i < NC; ++i) hv[i] = (double)i / NC;
 #define GEN_OR_RET(expr)                                   \
   do {                                                     \
     if ((expr) != hipSuccess) return GPP_EDEVICE;          \

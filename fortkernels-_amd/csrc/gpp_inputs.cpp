@@ -15,6 +15,7 @@
 // config 3: config 2 plus band-resolved energies e_n1 = sorted U(-20,20),
 //           wx[n1][iw] = 4 - e_n1 + (iw-1), and 5% of the wtilde real parts
 //           moved to within 1e-4 of a randomly chosen wx[n1][iw].
+// config | GPP_FILL_VCOUL_LINEAR: any of the above with vcoul[i] = i / NC.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -61,13 +62,13 @@ void parallel_for(size_t n, F f) {
 
 }  // namespace
 
-extern "C" int gpp_fill_inputs(int config, unsigned long long seed,
-                               int number_bands, int ncouls, int ngpown,
-                               int nodes_per_group, gpp_c128 *aqsmtemp,
-                               gpp_c128 *aqsntemp, gpp_c128 *I_eps_array,
-                               gpp_c128 *wtilde_array, double *vcoul,
-                               int *inv_igp_index, int *indinv,
-                               double *wx_array) {
+// The fills of one configuration (1, 2 or 3), as described above.
+static int fill_config(int config, unsigned long long seed, int number_bands,
+                       int ncouls, int ngpown, int nodes_per_group,
+                       gpp_c128 *aqsmtemp, gpp_c128 *aqsntemp,
+                       gpp_c128 *I_eps_array, gpp_c128 *wtilde_array,
+                       double *vcoul, int *inv_igp_index, int *indinv,
+                       double *wx_array) {
   const int NB = number_bands, NC = ncouls, NG = ngpown;
   if (config < 1 || config > 3 || NB < 0 || NC <= 0 || NG < 0)
     return GPP_EINVAL;
@@ -146,4 +147,21 @@ extern "C" int gpp_fill_inputs(int config, unsigned long long seed,
     });
   }
   return GPP_OK;
+}
+
+extern "C" int gpp_fill_inputs(int config, unsigned long long seed,
+                               int number_bands, int ncouls, int ngpown,
+                               int nodes_per_group, gpp_c128 *aqsmtemp,
+                               gpp_c128 *aqsntemp, gpp_c128 *I_eps_array,
+                               gpp_c128 *wtilde_array, double *vcoul,
+                               int *inv_igp_index, int *indinv,
+                               double *wx_array) {
+  const bool vlin = config & GPP_FILL_VCOUL_LINEAR;
+  const int rc = fill_config(config & ~GPP_FILL_VCOUL_LINEAR, seed,
+                             number_bands, ncouls, ngpown, nodes_per_group,
+                             aqsmtemp, aqsntemp, I_eps_array, wtilde_array,
+                             vcoul, inv_igp_index, indinv, wx_array);
+  if (rc == GPP_OK && vlin)
+    for (int i = 0; i < ncouls; ++i) vcoul[i] = (double)i / ncouls;
+  return rc;
 }

@@ -3,6 +3,12 @@
 //   gpp <number_bands> <number_valence_bands> <number_plane_waves>
 //       <nodes_per_mpi_group> [--precision=fp64|mixed|factored]
 //       [--semantics=gppker|strict|bgw] [--config=1|2|3] [--seed=N]
+//       [--vcoul=file|linear] [--tune] [--batch=N]
+//
+// --vcoul=linear fills vcoul[i] = i / ncouls instead of the configuration's
+// fill.  --tune runs the tile autotuner for the shape first (and prints the
+// tile it picked).  --batch=N evaluates N copies of the problem (seeds
+// seed .. seed+N-1) in one gpp_sigma_batch call and prints each result.
 //
 // Same positional arguments, usage text, banners and result lines as
 // gppKer.cpp:151-201,237-239,376-381; the band loop (:270-361) is replaced
@@ -44,6 +50,8 @@ int main(int argc, char **argv) {
   bool precision_given = false;
   int config = 1;
   unsigned long long seed = 12345;
+  bool vcoul_linear = false, tune = false;
+  int batch = 0;
   for (int i = 5; i < argc; ++i) {
     const std::string a = argv[i];
     auto val = [&](const char *k) -> const char * {
@@ -63,6 +71,21 @@ int main(int argc, char **argv) {
       config = std::atoi(v);
     } else if (const char *v = val("--seed=")) {
       seed = std::strtoull(v, nullptr, 10);
+    } else if (const char *v = val("--vcoul=")) {
+      if (std::strcmp(v, "linear") == 0)
+        vcoul_linear = true;
+      else if (std::strcmp(v, "file") != 0) {
+        std::cerr << "--vcoul takes file or linear" << std::endl;
+        return 2;
+      }
+    } else if (a == "--tune") {
+      tune = true;
+    } else if (const char *v = val("--batch=")) {
+      batch = std::atoi(v);
+      if (batch < 1) {
+        std::cerr << "--batch takes a positive count" << std::endl;
+        return 2;
+      }
     } else {
       std::cerr << "unknown option " << a << std::endl;
       return 2;
@@ -108,7 +131,8 @@ int main(int argc, char **argv) {
             << (ncouls * (double)ngpown * 2.0 * 8) / std::pow(1024, 2)
             << " Mbytes" << std::endl;
 
-  int rc = gpp_fill_inputs(config, seed, number_bands, ncouls, ngpown,
+  const int fill = config | (vcoul_linear ? GPP_FILL_VCOUL_LINEAR : 0);
+  int rc = gpp_fill_inputs(fill, seed, number_bands, ncouls, ngpown,
                            nodes_per_group, aqsm.data(), aqsn.data(),
                            ieps.data(), wt.data(), vcoul.data(), inv.data(),
                            indinv.data(), wx.data());
@@ -126,25 +150,81 @@ int main(int argc, char **argv) {
                    aqsm.data(),  aqsn.data(), ieps.data(), wt.data(),
                    vcoul.data(), inv.data(),  indinv.data(), wx.data(),
                    config == 3 ? 1 : 0};
+  auto fail = [&](const char *what, int code) {
+    std::cerr << "error: " << what << " failed (" << code
+              << "): " << gpp_last_error(h) << std::endl;
+    gpp_destroy(h);
+    return 4;
+  };
+  if (tune) {
+    int k = 0, w = 0;
+    if ((rc = gpp_autotune(h, &p, &opt, &k, &w))) return fail("gpp_autotune", rc);
+    std::cout << "tuned tile: " << k << " G-vectors x " << w
+              << " waves per block" << std::endl;
+  }
+  auto print = [&](const gpp_result &res, const char *tag) {
+    std::cout << tag << "achstemp = " << cx(res.achstemp) << std::endl;
+    for (int iw = nstart; iw < nend; ++iw)
+      std::cout << tag << "achtemp[" << iw << "] = " << std::setprecision(15)
+                << cx(res.achtemp[iw]) << std::endl;
+    for (int iw = nstart; iw < nend; ++iw)
+      std::cout << tag << "asxtemp[" << iw << "] = " << std::setprecision(15)
+                << cx(res.asxtemp[iw]) << std::endl;
+  };
+
+  if (batch > 0) {
+    // N problems of this shape, seeds seed .. seed+N-1 (config 1 ignores
+    // the seed, so its copies are identical).
+    std::vector<std::vector<gpp_c128>> bc(4 * (size_t)batch);
+    std::vector<std::vector<double>> bv((size_t)batch), bw((size_t)batch);
+    std::vector<std::vector<int>> bi((size_t)batch), bd((size_t)batch);
+    std::vector<gpp_problem> probs((size_t)batch, p);
+    for (int b = 0; b < batch; ++b) {
+      for (int j = 0; j < 4; ++j) bc[4 * b + j].resize(j < 2 ? nbnc : ngnc);
+      bv[b].resize(ncouls);
+      bw[b].resize(wx.size());
+      bi[b].resize(inv.size());
+      bd[b].resize(ncouls);
+      rc = gpp_fill_inputs(fill, seed + b, number_bands, ncouls, ngpown,
+                           nodes_per_group, bc[4 * b].data(),
+                           bc[4 * b + 1].data(), bc[4 * b + 2].data(),
+                           bc[4 * b + 3].data(), bv[b].data(), bi[b].data(),
+                           bd[b].data(), bw[b].data());
+      if (rc) return fail("gpp_fill_inputs", rc);
+      probs[b].aqsmtemp = bc[4 * b].data();
+      probs[b].aqsntemp = bc[4 * b + 1].data();
+      probs[b].I_eps_array = bc[4 * b + 2].data();
+      probs[b].wtilde_array = bc[4 * b + 3].data();
+      probs[b].vcoul = bv[b].data();
+      probs[b].inv_igp_index = bi[b].data();
+      probs[b].indinv = bd[b].data();
+      probs[b].wx_array = bw[b].data();
+    }
+    std::vector<gpp_result> res((size_t)batch);
+    std::memset(res.data(), 0, res.size() * sizeof(gpp_result));
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    rc = gpp_sigma_batch(h, probs.data(), batch, &opt, res.data());
+    const std::chrono::duration<double> el =
+        std::chrono::high_resolution_clock::now() - t0;
+    if (rc) return fail("gpp_sigma_batch", rc);
+    for (int b = 0; b < batch; ++b) {
+      const std::string tag = "[" + std::to_string(b) + "] ";
+      print(res[b], tag.c_str());
+    }
+    std::cout << "********** Chrono Time Taken **********= " << el.count()
+              << " secs" << std::endl;
+    gpp_destroy(h);
+    return 0;
+  }
+
   gpp_result res;
   std::memset(&res, 0, sizeof res);
   const auto t0 = std::chrono::high_resolution_clock::now();
   rc = gpp_sigma_ex(h, &p, &opt, &res);
   const std::chrono::duration<double> el =
       std::chrono::high_resolution_clock::now() - t0;
-  if (rc) {
-    std::cerr << "error: gpp_sigma_ex failed (" << rc
-              << "): " << gpp_last_error(h) << std::endl;
-    gpp_destroy(h);
-    return 4;
-  }
-  std::cout << "achstemp = " << cx(res.achstemp) << std::endl;
-  for (int iw = nstart; iw < nend; ++iw)
-    std::cout << "achtemp[" << iw << "] = " << std::setprecision(15)
-              << cx(res.achtemp[iw]) << std::endl;
-  for (int iw = nstart; iw < nend; ++iw)
-    std::cout << "asxtemp[" << iw << "] = " << std::setprecision(15)
-              << cx(res.asxtemp[iw]) << std::endl;
+  if (rc) return fail("gpp_sigma_ex", rc);
+  print(res, "");
   std::cout << "********** Chrono Time Taken **********= " << el.count()
             << " secs" << std::endl;
   std::cout << "Kernel time (non-occupied, occupied) = " << res.unocc_kernel_ms

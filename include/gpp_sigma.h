@@ -232,7 +232,11 @@ GPP_API int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
  * config: 1 = gppKer.cpp constant fills; 2 = canonical seeded random;
  * 3 = band-resolved energies with near-resonant wtilde (needs wx[NB][3]).
  * nodes_per_group gives the inv_igp_index stride of configs 2 and 3.
+ * config | GPP_FILL_VCOUL_LINEAR replaces vcoul by vcoul[i] = i / ncouls
+ * (a Coulomb factor proportional to i, in place of the file's 1.0 or the
+ * canonical 1/(0.01 + (i/NC)^(2/3))).
  * Deterministic for (config, seed, shape) on every host. */
+#define GPP_FILL_VCOUL_LINEAR 256
 GPP_API int gpp_fill_inputs(int config, unsigned long long seed, int number_bands,
                     int ncouls, int ngpown, int nodes_per_group,
                     gpp_c128 *aqsmtemp, gpp_c128 *aqsntemp,

@@ -435,3 +435,211 @@ def test_batch_result_view_matches_struct_fields():
         for k, v in ref.items():
             assert np.array_equal(np.asarray(fast[i][k]), np.asarray(v)), (i, k)
     assert gppk._RESULT_DTYPE.itemsize == C.sizeof(gppk.Result)
+
+
+# --------------------------------------------------------------------------
+# The frequency clamp (gppKer.cpp:279: `if (abs(wx) < to1) wx = to1`), the
+# linear Coulomb factor, and the bgw mask on a known subset of terms, by
+# closed forms of the constant fills.
+
+TO1 = 1e-6
+
+
+def clamp(w):
+    """gppKer.cpp:279 as written: any |wx| below to1 becomes +to1."""
+    return TO1 if abs(w) < TO1 else w
+
+
+def closed_form_rows(wx_rows, nv, ng, nc, vsum, sem="gppker"):
+    """Constant fills with band-resolved frequencies wx_rows[n1][iw] (before
+    the clamp) and sum_my_igp vcoul[igp] = vsum: every band adds
+    vsum nc c delw(wx)/4 to achtemp (the occupied ones through the solver's
+    first branch, which every wx here takes: |wx - c|^2 > 0.25 and
+    |delw|^2 < 2), each occupied band vsum nc ssx(wx)/2 to asxtemp (see
+    closed_form)."""
+    ach = np.zeros(3, complex)
+    asx = np.zeros(3, complex)
+    for n1, row in enumerate(wx_rows):
+        for iw in range(3):
+            wx = clamp(row[iw])
+            delw = C * np.conj(wx - C) / abs(wx - C) ** 2
+            ach[iw] += vsum * nc * 0.25 * C * delw
+            if n1 >= nv:
+                continue
+            if sem == "bgw":
+                cden = wx * wx - C * C
+                ssx = C ** 3 * np.conj(cden) / abs(cden) ** 2
+                cut = wx < 0 and abs(ssx) > 4 * abs(C)
+                asx[iw] -= 0.0 if cut else vsum * nc * 0.5 * ssx
+            else:
+                asx[iw] += vsum * nc * 0.5 * C ** 3 / wx ** 2
+    return ach, asx
+
+
+CLAMP_ROWS = [
+    (0.0, -5e-7, 5e-7),        # all three clamped to +1e-6
+    (1e-6, -1e-6, 2e-6),       # on and just past the threshold: unchanged
+    (-3e-7, 3.0, 9.99e-7),     # clamped, far, clamped
+    (3.0, 4.0, 5.0),           # the mini-app's grid
+]
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+@pytest.mark.parametrize("nv", [0, 2])
+def test_oracle_clamps_small_frequencies(sem, nv):
+    """Band-resolved rows with |wx| below, at and above 1e-6 in both
+    signs, occupied and non-occupied bands.  Planting a missing clamp, a
+    sign-keeping clamp (-5e-7 -> -1e-6) or `<=` for `<` (-1e-6 -> +1e-6)
+    into the closed form moves achtemp or asxtemp by >= 1e-6 relative."""
+    nb, nc, npg = len(CLAMP_ROWS), 64, 8
+    inp = gppk.fill_inputs(1, nb, nc, npg, nvband=nv)
+    inp["wx_array"] = np.array(CLAMP_ROWS, np.float64)
+    got = orc.sigma(inp, sem)
+    ach, asx = closed_form_rows(CLAMP_ROWS, nv, nc // npg, nc, nc // npg, sem)
+    assert rel_err(got["achtemp"], ach) < 1e-12, (got["achtemp"], ach)
+    if nv:
+        assert rel_err(got["asxtemp"], asx) < 1e-12, (got["asxtemp"], asx)
+    # the planted variants really are that far away
+    def planted(fn):
+        global clamp
+        keep = clamp
+        clamp = fn
+        try:
+            return closed_form_rows(CLAMP_ROWS, nv, nc // npg, nc, nc // npg, sem)
+        finally:
+            clamp = keep
+    for fn in (lambda w: w,
+               lambda w: np.copysign(TO1, w) if abs(w) < TO1 else w,
+               lambda w: TO1 if abs(w) <= TO1 else w):
+        try:
+            a2, s2 = planted(fn)
+        except ZeroDivisionError:  # unclamped wx = 0: ssx = c^3/0
+            continue
+        moved = rel_err(a2, ach) if not nv else max(rel_err(a2, ach), rel_err(s2, asx))
+        assert moved > 1e-7, moved
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_oracle_clamps_shared_frequencies(sem):
+    wx = np.array([0.0, -5e-7, 4.0])
+    inp = gppk.fill_inputs(1, 5, 64, 8, nvband=2)
+    inp["wx_array"] = wx.copy()
+    got = orc.sigma(inp, sem)
+    ach, asx = closed_form_rows([tuple(wx)] * 5, 2, 8, 64, 8, sem)
+    assert rel_err(got["achtemp"], ach) < 1e-12
+    assert rel_err(got["asxtemp"], asx) < 1e-12
+
+
+def linear_vcoul_sum(nc, ng):
+    """sum over my_igp of vcoul[igp] = igp/nc for the mini-app's strided
+    G' columns inv_igp_index[k] = (k+1) nc/ng, the last one (== nc) mapped to
+    nc-1 (gppKer.cpp:259-260,287-290)."""
+    s = 0.0
+    for k in range(ng):
+        igp = (k + 1) * nc // ng
+        if igp == nc:
+            igp = nc - 1
+        s += igp / nc
+    return s
+
+
+def test_fill_inputs_linear_vcoul():
+    for cfg in (1, 2, 3):
+        a = gppk.fill_inputs(cfg, 4, 96, 8, nvband=1, seed=9)
+        b = gppk.fill_inputs(cfg, 4, 96, 8, nvband=1, seed=9, vcoul_linear=True)
+        assert np.array_equal(b["vcoul"], np.arange(96) / 96.0)
+        for k in ("aqsmtemp", "aqsntemp", "I_eps_array", "wtilde_array",
+                  "inv_igp_index", "indinv", "wx_array"):
+            assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+@pytest.mark.parametrize("shape", [(64, 2, 2048, 8), (9, 3, 100, 7)])
+def test_oracle_linear_vcoul_closed_form(sem, shape):
+    """Config 1 with vcoul proportional to i: the closed form needs only
+    sum_my_igp vcoul[igp] (achtemp, asxtemp) and the gppker achstemp is
+    NB |c|^2 c sum vcoul / 2 (gppKer.cpp:96-97, the igp term only)."""
+    nb, nv, nc, npg = shape
+    ng = nc // npg
+    inp = gppk.fill_inputs(1, nb, nc, npg, nvband=nv, vcoul_linear=True)
+    vsum = linear_vcoul_sum(nc, ng)
+    assert vsum != ng  # not the file's vcoul = 1
+    got = orc.sigma(inp, sem)
+    ach, asx = closed_form_rows([(3.0, 4.0, 5.0)] * nb, nv, ng, nc, vsum, sem)
+    assert rel_err(got["achtemp"], ach) < 1e-12
+    assert rel_err(got["asxtemp"], asx) < 1e-12
+    if sem == "gppker":
+        want = nb * 0.5 * abs(C) ** 2 * C * vsum
+        assert abs(got["achstemp"] - want) <= 1e-12 * abs(want)
+
+
+def masked_problem(k_list):
+    """Constant fills, 3 non-occupied bands, wx = {3,4,5} shared, and
+    wtilde replaced at the (my_igp, ig) of k_list by w' (value given per
+    entry).  Returns the inputs and the per-term closed form: each band
+    adds  0.25 vcoul [ sum_all c delw_c(wx) + sum_placed (kept(w') c
+    delw_w'(wx) - c delw_c(wx)) ]  with kept() the bgw mask of
+    gppkernel.deslippe.09.30.2014.f90:364-366 (always true for gppker)."""
+    nb, nc, npg = 3, 64, 8
+    inp = gppk.fill_inputs(1, nb, nc, npg, nvband=0)
+    w = inp["wtilde_array"].copy()
+    for (g, ig, val) in k_list:
+        w[g, ig] = val
+    inp["wtilde_array"] = w
+    return inp
+
+
+def masked_expected(inp, k_list, sem):
+    nb = inp["aqsmtemp"].shape[0]
+    ng, nc = inp["I_eps_array"].shape
+    wx_rows = np.broadcast_to(inp["wx_array"], (nb, 3)) if inp["wx_array"].ndim == 1 \
+        else inp["wx_array"]
+    ach = np.zeros(3, complex)
+    dropped = 0
+    for n1 in range(nb):
+        for iw in range(3):
+            wx = clamp(wx_rows[n1][iw])
+            delw_c = C * np.conj(wx - C) / abs(wx - C) ** 2
+            tot = ng * nc * delw_c
+            for (_, _, wp) in k_list:
+                d = abs(wx - wp) ** 2
+                delw = wp * np.conj(wx - wp) / d
+                kept = sem != "bgw" or (d > 0.25 and abs(delw) ** 2 < 2.5e5)
+                dropped += not kept
+                tot += (delw if kept else 0.0) - delw_c
+            ach[iw] += 0.25 * C * tot
+    return ach, dropped
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_oracle_bgw_mask_drops_exactly_the_window_terms(sem):
+    """k = 3 wtilde placed inside the 0.5 window of wx = 4 (|wx - w'|^2 <=
+    0.25) and one far away: bgw drops exactly those 3 terms per band (one
+    frequency each), gppker keeps them; a mask dropping one term more or
+    fewer moves achtemp by ~1e-3 relative."""
+    placed = [(0, 5, 4.0 + 0.1j), (3, 17, 4.45 + 0.0j), (7, 63, 3.6 - 0.2j),
+              (2, 30, 8.0 + 0.05j)]
+    inp = masked_problem(placed)
+    got = orc.sigma(inp, sem)["achtemp"]
+    want, dropped = masked_expected(inp, placed, sem)
+    assert dropped == (3 * 3 if sem == "bgw" else 0)
+    assert rel_err(got, want) < 1e-12, (got, want)
+    # one dropped term more or fewer is far outside that tolerance
+    wp = placed[0][2]
+    one = 0.25 * C * (wp * np.conj(4.0 - wp) / abs(4.0 - wp) ** 2)
+    assert abs(one) / abs(want[1]) > 1e-4
+
+
+def test_oracle_bgw_mask_second_condition():
+    """A term with |wdiff|^2 > 0.25 but |delw|^2 = |wtilde|^2/|wdiff|^2 >=
+    2.5e5 (wtilde = 300, band-resolved wx = 300.55: |wdiff|^2 = 0.3025,
+    |delw|^2 = 2.98e5) is dropped by the limitone condition alone."""
+    placed = [(1, 9, 300.0 + 0.0j)]
+    inp = masked_problem(placed)
+    inp["wx_array"] = np.array([[300.55, 4.0, 5.0]] * 3)
+    got = orc.sigma(inp, "bgw")["achtemp"]
+    want, dropped = masked_expected(inp, placed, "bgw")
+    assert dropped == 3  # one per band, at iw = 0
+    assert rel_err(got, want) < 1e-12, (got, want)
+    keep, _ = masked_expected(inp, placed, "gppker")
+    assert rel_err(keep, want) > 1e-3
