@@ -563,16 +563,13 @@ def run_ff(args, name):
 
 
 BATCHES = {
-    # BASELINE's batch: 64 instances, ncouls log-uniform in [2048, 32768],
-    # number_bands in {128, 256, 512, 1024}, nvband in [2, 16], 20 per group
-    "batch": dict(n=64, nc=(2048, 32768), nb=(128, 256, 512, 1024), nv=(2, 16),
-                  npg=20, desc="64 independent GPP sigma evaluations, batched "
-                               "launches in a hipGraph, autotuned tile"),
+    # shapes: gppk.BATCH_WORKLOADS / gppk.draw_batch
+    "batch": dict(n=64, desc="64 independent GPP sigma evaluations, batched "
+                             "launches in a hipGraph, autotuned tile"),
     # many small instances, where per-launch costs and tails matter
-    "batch_small": dict(n=512, nc=(256, 2048), nb=(16, 32, 64), nv=(2, 8), npg=8,
-                        desc="512 small independent GPP sigma evaluations "
-                             "(ncouls 256-2048, 16-64 bands), batched launches "
-                             "in a hipGraph, autotuned tile"),
+    "batch_small": dict(n=512, desc="512 small independent GPP sigma evaluations "
+                                    "(ncouls 256-2048, 16-64 bands), batched "
+                                    "launches in a hipGraph, autotuned tile"),
 }
 
 
@@ -580,14 +577,8 @@ def run_batch(args, name="batch"):
     import torch
     gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
     w = BATCHES[name]
-    rng = np.random.default_rng(args.seed)
-    insts = []
-    for i in range(w["n"]):
-        nc = int(round(np.exp(rng.uniform(np.log(w["nc"][0]), np.log(w["nc"][1])))))
-        nb = int(rng.choice(list(w["nb"])))
-        nv = int(rng.integers(w["nv"][0], w["nv"][1] + 1))
-        insts.append(gppk.fill_inputs_device(3, nb, nc, w["npg"], nvband=nv,
-                                             seed=args.seed + 1000 + i))
+    insts = [gppk.fill_inputs_device(3, nb, nc, npg, nvband=nv, seed=s)
+             for nb, nv, nc, npg, s in gppk.draw_batch(name, args.seed)]
     torch.cuda.synchronize()
     terms = sum(d["number_bands"] * d["ngpown"] * d["ncouls"] * 3 for d in insts)
     gbytes = sum(16 * (2 * d["number_bands"] + 2 * d["ngpown"]) * d["ncouls"]

@@ -654,6 +654,42 @@ def ff_fill_inputs_device(config: int, number_bands: int, nvband: int,
     return d
 
 
+# The batched workload of the benchmark: 64 independent band-resolved
+# (config 3) instances, ncouls log-uniform in [2048, 32768], number_bands in
+# {128, 256, 512, 1024}, nvband uniform in [2, 16], 20 G-vectors per G'
+# column; and 512 small ones for the per-launch costs.
+BATCH_WORKLOADS = {
+    "batch": dict(n=64, nc=(2048, 32768), nb=(128, 256, 512, 1024), nv=(2, 16),
+                  npg=20),
+    "batch_small": dict(n=512, nc=(256, 2048), nb=(16, 32, 64), nv=(2, 8), npg=8),
+}
+
+
+def draw_batch(name: str, seed: int) -> list:
+    """The instance shapes of a batch workload for `seed`:
+    [(number_bands, nvband, ncouls, nodes_per_group, instance_seed), ...]."""
+    w = BATCH_WORKLOADS[name]
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(w["n"]):
+        nc = int(round(np.exp(rng.uniform(np.log(w["nc"][0]), np.log(w["nc"][1])))))
+        nb = int(rng.choice(list(w["nb"])))
+        nv = int(rng.integers(w["nv"][0], w["nv"][1] + 1))
+        out.append((nb, nv, nc, w["npg"], seed + 1000 + i))
+    return out
+
+
+def column_slice(inp: dict, ng: int) -> dict:
+    """The sub-problem of the first `ng` G' columns (my_igp < ng): the full
+    sums are sums over columns, so a column slice is checked on its own."""
+    d = dict(inp)
+    d["I_eps_array"] = inp["I_eps_array"][:ng]
+    d["wtilde_array"] = inp["wtilde_array"][:ng]
+    d["inv_igp_index"] = inp["inv_igp_index"][:ng]
+    d["ngpown"] = ng
+    return d
+
+
 def band_slice(inp: dict, n1_begin: int, n1_end: int) -> dict:
     """The sub-problem of bands [n1_begin, n1_end) (band additivity: the full
     sums are the sums of the sub-problems')."""
