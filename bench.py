@@ -19,8 +19,12 @@ The JSON line also carries:
                 traffic = HBM bytes per launch measured by rocprofv3 (from
                 profiles/traffic_*.json when present, else null);
   cpu_baseline  the reference mini-app (oracle/_ref/gppKer_xl, built from
-                gppKerOpenMPFor_xl.cpp) on a band subset of the same shape,
-                all host cores, its own chrono timer; terms/s;
+                gppKerOpenMPFor_xl.cpp: gppKer.cpp without ITT calls and with
+                its non-occupied mask active, which keeps every term of its
+                constant fills) on a band subset of the same shape, with the
+                host threads the environment allots (OMP_NUM_THREADS, else
+                the affinity size), median of 3 runs of its own chrono timer;
+                terms/s, with the host's CPU model, nproc and affinity;
   host_inputs   the FP64 evaluation called with pageable host arrays (the
                 reference's calling convention), PCIe copies included.
   factored      the band-sum-first FP64 path (precision fp64_factored, only
@@ -129,12 +133,55 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_cores() -> int:
+def host_info() -> dict:
+    """What the CPU baseline runs on: the cores this process may use
+    (affinity), os.cpu_count(), physical cores and the CPU model from
+    /proc/cpuinfo, and the thread count used (OMP_NUM_THREADS when the
+    environment sets it -- the GPU pool exports its per-GPU CPU share there,
+    16 -- else the affinity size)."""
     try:
-        n = len(os.sched_getaffinity(0))
+        affinity = len(os.sched_getaffinity(0))
     except Exception:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        affinity = os.cpu_count() or 1
+    model, phys = "unknown", set()
+    try:
+        pid = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name":
+                model = v
+            elif k == "physical id":
+                pid = v
+            elif k == "core id":
+                core = v
+            elif not k and pid is not None and core is not None:
+                phys.add((pid, core))
+                pid = core = None
+        if pid is not None and core is not None:
+            phys.add((pid, core))
+    except OSError:
+        pass
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = int(env) if env.isdigit() and int(env) > 0 else affinity
+    threads = max(1, min(threads, affinity))
+    return {"threads": threads, "affinity": affinity, "nproc": os.cpu_count(),
+            "physical_cores": len(phys) or None, "cpu_model": model,
+            "omp_num_threads_env": env or None}
+
+
+def host_cores() -> int:
+    return host_info()["threads"]
+
+
+# The reference binary of the CPU baseline: gppKerOpenMPFor_xl.cpp built
+# unmodified.  It is gppKer.cpp without the Intel ITT calls and with the
+# non-occupied mask of :335-338 active (gppKer.cpp comments it out); for its
+# constant fills the mask is true for every term, so the work and the
+# printed sums are those of gppKer.cpp.
+CPU_REF_DESC = ("reference gppKerOpenMPFor_xl.cpp binary (gppKer.cpp without ITT "
+                "calls, non-occupied mask of :335-338 active; its constant fills, "
+                "where the mask keeps every term)")
 
 
 def rel_err(a, b) -> float:
@@ -144,9 +191,12 @@ def rel_err(a, b) -> float:
 
 
 def cpu_baseline(args):
-    """Reference binary on a band subset of the canonical shape."""
+    """Reference binary on a band subset of the canonical shape (terms
+    scale linearly in bands), median of three runs of its own chrono timer
+    around the band loop (gppKer.cpp:266,375)."""
     exe = ROOT / "oracle" / "_ref" / "gppKer_xl"
-    cores = host_cores()
+    info = host_info()
+    cores = info["threads"]
     ng = NC // NPG
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread",
                OMP_PLACES="cores")
@@ -159,14 +209,18 @@ def cpu_baseline(args):
             return t, out
         t4, _ = run(4)
         rate = 4 * ng * NC * 3 / t4
-        nb = int(max(4, min(NB, round(args.cpu_seconds * rate / (ng * NC * 3)))))
-        t, out = run(nb)
+        # three runs of about cpu_seconds/3 each
+        nb = int(max(4, min(NB, round(args.cpu_seconds / 3 * rate / (ng * NC * 3)))))
+        times = sorted(run(nb)[0] for _ in range(3))
+        t = times[1]
         terms = nb * ng * NC * 3
         return {"value": terms / t, "unit": "terms/s", "cores": cores,
                 "kind": "reference",
-                "sample": f"reference gppKerOpenMPFor_xl.cpp binary, {nb} {NV} {NC} {NPG} "
-                          f"(band subset of 512 {NV} {NC} {NPG}; its constant fills), "
-                          f"OMP_NUM_THREADS={cores}, chrono {t:.2f} s"}
+                "sample": f"{CPU_REF_DESC}, {nb} {NV} {NC} {NPG} (band subset of "
+                          f"512 {NV} {NC} {NPG}), OMP_NUM_THREADS={cores} "
+                          f"OMP_PROC_BIND=spread OMP_PLACES=cores; median of 3 "
+                          f"chrono times {', '.join(f'{x:.2f}' for x in times)} s",
+                "runs_s": times, "host": info}
     # Port fallback: the oracle (compensated sums, so slower than the reference).
     gppk = _load("gppk", ROOT / "fortkernels-_amd" / "__init__.py", pkg=True)
     orc = _load("gpp_oracle", ROOT / "oracle" / "oracle.py")
@@ -175,7 +229,8 @@ def cpu_baseline(args):
     orc.sigma(inp, want_achstemp=False)
     t = time.perf_counter() - t0
     return {"value": 3 * ng * NC * 3 / t, "unit": "terms/s", "cores": cores,
-            "kind": "port", "sample": f"CPU oracle, 3 bands of 512 {NV} {NC} {NPG}"}
+            "kind": "port", "sample": f"CPU oracle, 3 bands of 512 {NV} {NC} {NPG}",
+            "host": info}
 
 
 def to_host(d: dict) -> dict:
@@ -437,7 +492,8 @@ def ff_cpu_baseline(args, shape):
     if not exe.exists():
         return None
     nb, nv, nc, ng, nf, ne = shape
-    cores = host_cores()
+    info = host_info()
+    cores = info["threads"]
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread",
                OMP_PLACES="cores")
     nf_s = min(nf, 8 if nf <= 1000 else 2000)
@@ -460,7 +516,8 @@ def ff_cpu_baseline(args, shape):
             "sample": f"FFKer_cori/ffKernel.cpp binary (its constant fills) "
                       f"{nb} {nv} {nc} {ng} {nf_s} {ne_s}, OMP_NUM_THREADS={cores}; "
                       f"kernel timers " + ", ".join(f"{k} {t[k]:.3f} s" for k in names) +
-                      f"; scaled to {' '.join(map(str, shape))}: {est:.2f} s per evaluation"}
+                      f"; scaled to {' '.join(map(str, shape))}: {est:.2f} s per evaluation",
+            "host": info}
 
 
 def run_ff(args, name):

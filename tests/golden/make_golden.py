@@ -5,7 +5,9 @@ Run in a container that has the reference tree at /root/reference:
     python tests/golden/make_golden.py
 
 1. ref_constfill.json — the achtemp lines printed by the reference mini-app
-   gppKerOpenMPFor_xl.cpp (gppKer.cpp without the Intel ITT calls), built
+   gppKerOpenMPFor_xl.cpp (gppKer.cpp without the Intel ITT calls and with
+   the non-occupied mask of :335-338 active, which keeps every term of the
+   constant fills), built
    unmodified by oracle/Makefile, for several shapes with its own constant
    fills (gppKer.cpp:241-264).  OMP_NUM_THREADS=1 so that the printed sums
    are in one fixed order.
