@@ -45,6 +45,7 @@ FLAG_SKIP_UNOCC = 2
 FLAG_CHECK_FINITE = 4
 FLAG_NO_ACHSTEMP = 8
 FLAG_AUTOTUNE = 16
+FLAG_MIXED_KAHAN = 32
 
 SEMANTICS = {"gppker": SEM_GPPKER, "strict": SEM_GPPKER_STRICT, "bgw": SEM_BGW}
 PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED,

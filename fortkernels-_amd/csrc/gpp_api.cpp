@@ -164,9 +164,6 @@ int validate(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
     return set_err(h, GPP_EINVAL, "bad semantics %d", o->semantics);
   if (o->memory != GPP_MEM_HOST && o->memory != GPP_MEM_DEVICE)
     return set_err(h, GPP_EINVAL, "bad memory space %d", o->memory);
-  if (o->precision == GPP_PREC_MIXED && o->semantics == GPP_SEM_BGW)
-    return set_err(h, GPP_EUNSUPPORTED,
-                   "mixed precision supports the gppker semantics only");
   if (o->semantics == GPP_SEM_GPPKER_STRICT && p->ngpown > p->ncouls)
     return set_err(h, GPP_EINVAL,
                    "strict semantics index I_eps[my_igp*ngpown+ig] needs "
@@ -194,6 +191,14 @@ int validate(gpp_handle h, const gpp_problem *p, const gpp_options *o) {
 namespace {
 
 using gpp::validate;
+
+// The precision argument of the launchers: the public one, or the mixed
+// path with compensated run sums.
+int kernel_precision(const gpp_options *o) {
+  return (o->precision == GPP_PREC_MIXED && (o->flags & GPP_FLAG_MIXED_KAHAN))
+             ? gpp::kPrecMixedKahan
+             : o->precision;
+}
 
 int ilog2_ceil(long long x) {
   int l = 0;
@@ -312,9 +317,14 @@ int prepare(gpp_handle h, const gpp_problem *p, int n, const gpp_options *o,
     const size_t nbnc = (size_t)p[i].number_bands * p[i].ncouls;
     const size_t nbng = (size_t)p[i].number_bands * p[i].ngpown;
     woff[(size_t)i * 4 + 0] = wa.take((size_t)p[i].ngpown * sizeof(int));
-    // wx [NB][3] (clamped), then the occupied solver's factor [NB][3]
-    woff[(size_t)i * 4 + 1] = wa.take((size_t)p[i].number_bands * 6 * 8);
-    woff[(size_t)i * 4 + 2] = mixed ? wa.take(nbnc * sizeof(float2)) : 0;
+    // wx [NB][3] (clamped), the occupied solver's factor [NB][3], the FP32
+    // split of wx [NB][3][2]
+    woff[(size_t)i * 4 + 1] =
+        wa.take((size_t)p[i].number_bands * gpp::kWxDoubles * 8);
+    woff[(size_t)i * 4 + 2] =
+        mixed ? wa.take((size_t)p[i].number_bands *
+                        gpp::padded_nc(p[i].ncouls) * sizeof(float2))
+              : 0;
     woff[(size_t)i * 4 + 3] = wa.take(nbng * 16);
   }
   if ((rc = ensure(h, h->work, wa.used))) return rc;
@@ -385,9 +395,8 @@ int enqueue_prepass(gpp_handle h, const Prepared &q, bool mixed,
                                         q.dp.NC, q.dp.NG,
                                         (double2 *)q.dp.aqsmG, s));
   if (mixed && q.dp.NB > q.dp.NV)
-    HIP_OR_RET(h, gpp::launch_aqsn_to_fp32(
-                      q.dp.aqsn, (float2 *)q.dp.aqsn32,
-                      (size_t)q.dp.NB * q.dp.NC, s));
+    HIP_OR_RET(h, gpp::launch_aqsn_to_fp32(q.dp.aqsn, (float2 *)q.dp.aqsn32,
+                                           q.dp.NB, q.dp.NC, s));
   return GPP_OK;
 }
 
@@ -622,7 +631,7 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
                                                o->semantics, n1u, s, g, cnt));
       }
       else
-        HIP_OR_RET(h, gpp::launch_unocc(dp, n1u_begin, NB, o->precision,
+        HIP_OR_RET(h, gpp::launch_unocc(dp, n1u_begin, NB, kernel_precision(o),
                                         o->semantics, tile, thr, part, n1u,
                                         mixed ? dcount : nullptr, s, g, cnt));
     }
@@ -883,7 +892,7 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     const DevProblem &d = prep[i].dp;
     max_ng_nb3 = std::max(max_ng_nb3, std::max(d.NG, 3 * d.NB));
     max_nbng = std::max(max_nbng, (size_t)d.NB * d.NG);
-    max_nbnc = std::max(max_nbnc, (size_t)d.NB * d.NC);
+    max_nbnc = std::max(max_nbnc, (size_t)d.NB * gpp::padded_nc(d.NC));
     max_ach = std::max(max_ach, ach_count[i]);
   }
   Arena ma;
@@ -955,7 +964,7 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
       const BatchItem *ditems = (const BatchItem *)(mb + off_items);
       hipError_t e = gpp::launch_batch(
           (const DevProblem *)(mb + off_tab), ditems, n_u, ditems + n_u, n_o,
-          tile.k_ig, o->precision, o->semantics, thr, part, dcount,
+          tile.k_ig, kernel_precision(o), o->semantics, thr, part, dcount,
           64 * tile.waves, (const int *)(mb + off_rb),
           (const int *)(mb + off_rc), n, dout, s);
       if (e != hipSuccess)

@@ -179,6 +179,13 @@ __device__ __forceinline__ void prepass_one(int i, const int *__restrict__ inv_i
     // cden = wxt^2 (gppKer.cpp:38-39), once per (band, frequency)
     const double c = w * w;
     wx_out[NB * 3 + i] = c * recip(c * c);
+    // FP32 hi/lo split of the frequency for the mixed-precision kernel,
+    // 8 floats (32 bytes) per band
+    float *split = (float *)(wx_out + NB * 6) + (i / 3) * 8 + (i % 3) * 2;
+    const float hi = (float)w;
+    split[0] = hi;
+    split[1] = (float)(w - (double)hi);
+    if (i % 3 == 2) split[2] = split[3] = 0.0f;
   }
 }
 
@@ -215,14 +222,29 @@ __global__ void gather_aqsm_kernel(const double2 *__restrict__ aqsm,
   }
 }
 
-__global__ void to_fp32_kernel(const double2 *__restrict__ in,
-                               float2 *__restrict__ out, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) {
-    const double2 v = in[i];
-    out[i] = make_float2((float)v.x, (float)v.y);
+// out[n1][ig] = (float2)in[n1][ig] with rows padded to padded_nc(NC) by
+// zeros.
+__device__ __forceinline__ void to_fp32_rows(const double2 *__restrict__ in,
+                                             float2 *__restrict__ out, int NB,
+                                             int NC, size_t i0, size_t stride) {
+  const int ncp = padded_nc(NC);
+  const size_t n = (size_t)NB * ncp;
+  for (size_t i = i0; i < n; i += stride) {
+    const int ig = (int)(i % ncp);
+    const size_t n1 = i / ncp;
+    float2 o = make_float2(0.0f, 0.0f);
+    if (ig < NC) {
+      const double2 v = in[n1 * NC + ig];
+      o = make_float2((float)v.x, (float)v.y);
+    }
+    out[i] = o;
   }
+}
+
+__global__ void to_fp32_kernel(const double2 *__restrict__ in,
+                               float2 *__restrict__ out, int NB, int NC) {
+  to_fp32_rows(in, out, NB, NC, (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+               (size_t)gridDim.x * blockDim.x);
 }
 
 // The same three steps for every problem of a batch in one launch each
@@ -256,14 +278,9 @@ __global__ void gather_aqsm_batch_kernel(const DevProblem *__restrict__ probs) {
 __global__ void to_fp32_batch_kernel(const DevProblem *__restrict__ probs) {
   const DevProblem &p = probs[blockIdx.y];
   if (p.NB <= p.NV || !p.aqsn32) return;
-  const size_t n = (size_t)p.NB * p.NC;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  float2 *out = (float2 *)p.aqsn32;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += stride) {
-    const double2 v = p.aqsn[i];
-    out[i] = make_float2((float)v.x, (float)v.y);
-  }
+  to_fp32_rows(p.aqsn, (float2 *)p.aqsn32, p.NB, p.NC,
+               (size_t)blockIdx.x * blockDim.x + threadIdx.x,
+               (size_t)gridDim.x * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -536,19 +553,83 @@ __global__ __launch_bounds__(256) __attribute__((
 // ---------------------------------------------------------------------------
 // Non-occupied bands, mixed precision.
 // Same decomposition as the FP64 kernel; the per-term work is FP32:
-//   dr = (xh - wrh) + (xl - wrl)      wx and wtilde.re split into FP32
-//                                     hi/lo pairs, so the cancellation
-//                                     near resonance is exact
+//   dr = (xh - wrh) + (xl - wrl)      wx (split once per band by the
+//                                     prepass) and wtilde.re as FP32 hi/lo
+//                                     pairs, so the cancellation near
+//                                     resonance is exact
 //   d  = dr^2 + wi^2, r = v_rcp_f32(d), B = aqsn * A, S += B*conj(wdiff)*r
-// with the FP32 run sum S over K G-vectors; conj(aqsm)*S and all later sums
-// are FP64.  If any term of a lane's run has d < thresh (near or exact
-// resonance, FP32 underflow), that lane's whole run for this band is redone
-// in FP64 from the FP64 arrays and replaces the FP32 sum.
-template <int K, bool N1LOG, typename Prob>
+// with the FP32 run sum S over the lane's K G-vectors (Kahan-compensated
+// when KAHAN, the compensation folded in when S is widened); conj(aqsm)*S
+// and all later sums are FP64.  On gfx950 an FP32 VALU instruction issues in half
+// the cycles of an FP64 one when two waves per SIMD are ready, so the band
+// loop keeps its next band's operands (the aqsn run and the split
+// frequencies, wave-uniform: scalar loads) in flight a whole band ahead.
+//
+// A lane's run for a band is redone in FP64 when one of its terms is near
+// resonance (|wdiff|^2 below thresh: FP32 underflow, exact resonance, or
+// the large terms whose FP32 error would dominate the sum) or, for the bgw
+// mask, when a term's |wdiff|^2 lies within 4e-6 (relative) of the mask's
+// threshold, where FP32 could decide it differently from FP64.  Such runs
+// are not redone in place (that would stall the whole wave on one lane's
+// FP64 work): the lane drops its FP32 run sum and appends (band, lane) to a
+// per-wave work list in LDS; whenever 64 entries are waiting, the wave
+// evaluates 64 runs in FP64, one per lane, and adds each one's
+// wgt*conj(aqsm)*S to the evaluating lane's correction sums (the owner's
+// weight is shared through LDS).  The list is drained at the end; with the
+// per-band log it is drained every band, so each band's log holds its own
+// corrections.  The order of the list is fixed by the data, so results are
+// reproducible run to run.
+constexpr int kMixQueue = 128;  // entries per wave: < 64 left + 64 new
+constexpr float kMaskBand = 4e-6f;  // relative band around the bgw threshold
+
+// The FP64 run of one work-list entry, weighted: wgt * conj(aqsm) * S64.
+template <int K, bool MASK, typename Prob>
+__device__ __forceinline__ void mixed_fp64_run(const Prob &p, int n1, int my_igp,
+                                               int ig0, double wo,
+                                               double (&cr)[3], double (&ci)[3]) {
+  const int NC = p.NC;
+  double sr[3] = {0.0, 0.0, 0.0}, si[3] = {0.0, 0.0, 0.0};
+  const double x[3] = {p.wx[n1 * 3 + 0], p.wx[n1 * 3 + 1], p.wx[n1 * 3 + 2]};
+  const auto an_row = p.aqsn + (size_t)n1 * NC;
+#pragma unroll 1
+  for (int k = 0; k < K; ++k) {
+    const int ig = ig0 + k;
+    if (ig >= NC) break;
+    const size_t off = (size_t)my_igp * NC + ig;
+    const double2 w = ld(p.wtilde + off);
+    const double2 e = ld(p.ieps + off);
+    const double2 a = ld(an_row + ig);
+    const double A_r = w.x * e.x - w.y * e.y;
+    const double A_i = w.x * e.y + w.y * e.x;
+    const double B_r = a.x * A_r - a.y * A_i;
+    const double B_i = a.x * A_i + a.y * A_r;
+    const double wy2 = w.y * w.y;
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      const double ddr = x[iw] - w.x;
+      const double dd = fma(ddr, ddr, wy2);
+      double rr = 1.0 / dd;
+      if (MASK)
+        rr = (dd > kLimitTwo && (w.x * w.x + wy2) * rr < kLimitOne) ? rr : 0.0;
+      sr[iw] += (B_r * ddr - B_i * w.y) * rr;
+      si[iw] += (B_r * w.y + B_i * ddr) * rr;
+    }
+  }
+  const double2 am = ld(p.aqsmG + (size_t)n1 * p.NG + my_igp);
+#pragma unroll
+  for (int iw = 0; iw < 3; ++iw) {
+    cr[iw] += wo * fma(am.x, sr[iw], am.y * si[iw]);
+    ci[iw] += wo * fma(am.x, si[iw], -am.y * sr[iw]);
+  }
+}
+
+template <int K, bool MASK, bool N1LOG, bool KAHAN, typename Prob>
 __device__ __forceinline__ void unocc_mixed_block(
     const Prob &p, int n1_begin, int n1_end, int b,
     double *__restrict__ rec, double *__restrict__ n1log, float thresh,
     unsigned long long *__restrict__ fallback_count) {
+  __shared__ unsigned queue_all[4][kMixQueue];
+  __shared__ double wgt_all[4][kWave];
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = uniform(threadIdx.x / kWave);
   const int W = blockDim.x / kWave;
@@ -558,11 +639,16 @@ __device__ __forceinline__ void unocc_mixed_block(
   const int ig_grp = b / ngrp;
   const int my_igp = igp_grp * kWave + lane;
   const int ig0 = uniform((ig_grp * W + wave) * K);
+  unsigned *queue = queue_all[wave];
+  double *wgts = wgt_all[wave];
 
   int igp = (my_igp < NG) ? p.igp_of[my_igp] : -1;
   const bool valid = igp >= 0;
+  const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
+  wgts[lane] = wgt;  // read by the lanes that evaluate this lane's entries
 
   float wrh[K], wrl[K], wi[K], wi2[K], Ar[K], Ai[K];
+  float idm[K];  // MASK: 1 / the mask threshold max(limittwo, |wtilde|^2/limitone)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int ig = ig0 + k;
@@ -575,57 +661,109 @@ __device__ __forceinline__ void unocc_mixed_block(
       wi[k] = (float)w.y;
       Ar[k] = (float)(w.x * e.x - w.y * e.y);
       Ai[k] = (float)(w.x * e.y + w.y * e.x);
+      if (MASK)
+        idm[k] = (float)(1.0 / fmax(kLimitTwo, (w.x * w.x + w.y * w.y) / kLimitOne));
     } else {
       // Inert slot: A = 0 and a far-away wtilde: |wdiff|^2 overflows to
-      // +inf, r = 0, contribution 0, never below the threshold.
+      // +inf, r = 0, contribution 0, never near a threshold.
       wrh[k] = 1e30f;
       wrl[k] = 0.0f;
       wi[k] = 0.0f;
       Ar[k] = 0.0f;
       Ai[k] = 0.0f;
+      if (MASK) idm[k] = 0.0f;
     }
     wi2[k] = wi[k] * wi[k];
   }
 
-  const double wgt = valid ? 0.5 * p.vcoul[igp] : 0.0;
+  // weighted sums: wgt * conj(aqsm) * S of the lane's FP32 runs plus the
+  // weighted FP64 runs this lane evaluated from the work list
   double acc_r[3] = {0.0, 0.0, 0.0}, acc_i[3] = {0.0, 0.0, 0.0};
   unsigned long long nfb = 0;
+  int qn = 0;  // entries waiting in this wave's list (wave-uniform)
   const auto aqsm_col = p.aqsmG + min(my_igp, NG - 1);
+  // the prepass's FP32 split of the frequencies, per band
+  // [xh0 xl0 xh1 xl1 xh2 xl2 0 0]; the zero-padded FP32 aqsn rows
+  const auto xsplit = (decltype(p.aqsn32))(p.wx + 6 * (size_t)p.NB);
+  const int ncp = padded_nc(NC);
   double2 am_next = (n1_begin < n1_end)
                         ? ld(aqsm_col + (size_t)n1_begin * NG)
                         : make_double2(0.0, 0.0);
+  float2 an_cur[K], xs_cur[3];
+  if (n1_begin < n1_end) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      an_cur[k] = ld(p.aqsn32 + ((size_t)n1_begin * ncp + ig0 + k));
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) xs_cur[iw] = ld(xsplit + n1_begin * 4 + iw);
+  }
+
+  // Evaluates `cnt` (<= 64) entries from the top of the list in FP64, one
+  // per lane, into the evaluating lanes' correction sums.
+  // The lanes of a wave execute their LDS accesses in order and every read
+  // here may alias the writes before it, so no fence is needed (a fence
+  // would also stop the compiler from turning the wave-uniform aqsn loads
+  // of the band loop into scalar loads).
+  auto drain = [&](int cnt) {
+    if (lane < cnt) {
+      const unsigned item = queue[qn - cnt + lane];
+      const int owner = (int)(item & (kWave - 1));
+      const int n1 = (int)(item >> 6);
+      mixed_fp64_run<K, MASK>(p, n1, igp_grp * kWave + owner, ig0,
+                              wgts[owner], acc_r, acc_i);
+      nfb += 3 * K;
+    }
+    qn -= cnt;
+  };
 
   for (int n1 = n1_begin; n1 < n1_end; ++n1) {
     const double2 am = am_next;
-    am_next = ld(aqsm_col + (size_t)min(n1 + 1, n1_end - 1) * NG);
-    const double x[3] = {p.wx[n1 * 3 + 0], p.wx[n1 * 3 + 1],
-                         p.wx[n1 * 3 + 2]};
-    float xh[3], xl[3];
+    const int n1n = min(n1 + 1, n1_end - 1);
+    am_next = ld(aqsm_col + (size_t)n1n * NG);
+    float2 an_nxt[K], xs_nxt[3];
 #pragma unroll
-    for (int iw = 0; iw < 3; ++iw) {
-      xh[iw] = (float)x[iw];
-      xl[iw] = (float)(x[iw] - (double)xh[iw]);
-    }
-    const auto an_row = p.aqsn32 + (size_t)n1 * NC;
+    for (int k = 0; k < K; ++k)
+      an_nxt[k] = ld(p.aqsn32 + ((size_t)n1n * ncp + ig0 + k));
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) xs_nxt[iw] = ld(xsplit + n1n * 4 + iw);
 
     float s_r[3] = {0.f, 0.f, 0.f}, s_i[3] = {0.f, 0.f, 0.f};
-    float dmin = 3.0e38f;
+    float c_r[3] = {0.f, 0.f, 0.f}, c_i[3] = {0.f, 0.f, 0.f};  // KAHAN
+    float dmin = 3.0e38f, bmin = 3.0e38f;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int igk = min(ig0 + k, NC - 1);
-      const float2 an = ld(an_row + igk);
-      const float Br = an.x * Ar[k] - an.y * Ai[k];
-      const float Bi = an.x * Ai[k] + an.y * Ar[k];
+      const float2 an = an_cur[k];
+      const float Br = fmaf(an.x, Ar[k], -an.y * Ai[k]);
+      const float Bi = fmaf(an.x, Ai[k], an.y * Ar[k]);
       const float nBiwi = -Bi * wi[k];
       const float Brwi = Br * wi[k];
       float d[3];
 #pragma unroll
       for (int iw = 0; iw < 3; ++iw) {
-        const float dr = (xh[iw] - wrh[k]) + (xl[iw] - wrl[k]);
+        const float dr = (xs_cur[iw].x - wrh[k]) + (xs_cur[iw].y - wrl[k]);
         d[iw] = fmaf(dr, dr, wi2[k]);
-        const float r = rcp_f32(d[iw]);
-        s_r[iw] = fmaf(fmaf(Br, dr, nBiwi), r, s_r[iw]);
-        s_i[iw] = fmaf(fmaf(Bi, dr, Brwi), r, s_i[iw]);
+        float r = rcp_f32(d[iw]);
+        if (MASK) {
+          // keep the term iff |wdiff|^2 > max(limittwo, |wtilde|^2 /
+          // limitone) (the f90 mask, :364-366); track the distance to
+          // that threshold so that runs near it are decided in FP64
+          const float rel = fmaf(d[iw], idm[k], -1.0f);
+          r = rel > 0.0f ? r : 0.0f;
+          bmin = fminf(bmin, fabsf(rel));
+        }
+        if (KAHAN) {
+          // compensated run sum: c holds minus the low part lost so far
+          const float tr = fmaf(Br, dr, nBiwi) * r - c_r[iw];
+          const float ti = fmaf(Bi, dr, Brwi) * r - c_i[iw];
+          const float ur = s_r[iw] + tr, ui = s_i[iw] + ti;
+          c_r[iw] = (ur - s_r[iw]) - tr;
+          c_i[iw] = (ui - s_i[iw]) - ti;
+          s_r[iw] = ur;
+          s_i[iw] = ui;
+        } else {
+          s_r[iw] = fmaf(fmaf(Br, dr, nBiwi), r, s_r[iw]);
+          s_i[iw] = fmaf(fmaf(Bi, dr, Brwi), r, s_i[iw]);
+        }
       }
       dmin = fminf(dmin, fminf(d[0], fminf(d[1], d[2])));
     }
@@ -634,55 +772,53 @@ __device__ __forceinline__ void unocc_mixed_block(
     for (int iw = 0; iw < 3; ++iw) {
       sr[iw] = (double)s_r[iw];
       si[iw] = (double)s_i[iw];
-    }
-    if (__builtin_expect(!(dmin >= thresh), 0)) {
-      // Rare: the lane's run is redone in FP64 (rolled loop, reloads, so
-      // that it adds no register pressure to the loop above).
-      sr[0] = sr[1] = sr[2] = si[0] = si[1] = si[2] = 0.0;
-      const auto an64 = p.aqsn + (size_t)n1 * NC;
-#pragma unroll 1
-      for (int k = 0; k < K; ++k) {
-        const int ig = ig0 + k;
-        if (!valid || ig >= NC) continue;
-        const size_t off = (size_t)my_igp * NC + ig;
-        const double2 w = ld(p.wtilde + off);
-        const double2 e = ld(p.ieps + off);
-        const double2 a = ld(an64 + ig);
-        const double A_r = w.x * e.x - w.y * e.y;
-        const double A_i = w.x * e.y + w.y * e.x;
-        const double B_r = a.x * A_r - a.y * A_i;
-        const double B_i = a.x * A_i + a.y * A_r;
-#pragma unroll
-        for (int iw = 0; iw < 3; ++iw) {
-          const double ddr = x[iw] - w.x;
-          const double dd = ddr * ddr + w.y * w.y;
-          const double rr = 1.0 / dd;
-          sr[iw] += (B_r * ddr - B_i * w.y) * rr;
-          si[iw] += (B_r * w.y + B_i * ddr) * rr;
-        }
+      if (KAHAN) {
+        sr[iw] -= (double)c_r[iw];
+        si[iw] -= (double)c_i[iw];
       }
-      nfb += 3 * K;
     }
-    double lr = 0.0, li = 0.0;
+    const bool near = !(dmin >= thresh) || (MASK && !(bmin >= kMaskBand));
+    const unsigned long long m = __ballot(near);
+    if (m) {  // wave-uniform
+      if (near) {
+        sr[0] = sr[1] = sr[2] = si[0] = si[1] = si[2] = 0.0;
+        const int pos = qn + (int)__builtin_amdgcn_mbcnt_hi(
+                                  (unsigned)(m >> 32),
+                                  __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        queue[pos] = ((unsigned)n1 << 6) | (unsigned)lane;
+      }
+      qn += __popcll(m);
+    }
+    const double wr = wgt * am.x, wi_ = wgt * am.y;  // wgt * aqsm
 #pragma unroll
     for (int iw = 0; iw < 3; ++iw) {
-      acc_r[iw] = fma(am.x, sr[iw], fma(am.y, si[iw], acc_r[iw]));
-      acc_i[iw] = fma(am.x, si[iw], fma(-am.y, sr[iw], acc_i[iw]));
-      if (iw == 2) {
-        lr = wgt * fma(am.x, sr[iw], am.y * si[iw]);
-        li = wgt * fma(am.x, si[iw], -am.y * sr[iw]);
-      }
+      acc_r[iw] = fma(wr, sr[iw], fma(wi_, si[iw], acc_r[iw]));
+      acc_i[iw] = fma(wr, si[iw], fma(-wi_, sr[iw], acc_i[iw]));
     }
-    if (N1LOG)
+    if (N1LOG) {
+      // this band's log: the lane's FP32 run plus the FP64 runs of this
+      // band, drained now (each evaluating lane adds what it evaluated)
+      const double c2r = acc_r[2], c2i = acc_i[2];
+      while (qn > 0) drain(min(qn, kWave));
+      const double lr = fma(wr, sr[2], wi_ * si[2]) + (acc_r[2] - c2r);
+      const double li = fma(wr, si[2], -wi_ * sr[2]) + (acc_i[2] - c2i);
       n1log_store(lr, li, n1log,
                   (size_t)b * (n1_end - n1_begin) + (n1 - n1_begin));
+    } else if (qn >= kWave) {
+      drain(kWave);
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) an_cur[k] = an_nxt[k];
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) xs_cur[iw] = xs_nxt[iw];
   }
+  while (qn > 0) drain(min(qn, kWave));
 
   double v[6];
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {
-    v[2 * iw] = wgt * acc_r[iw];
-    v[2 * iw + 1] = wgt * acc_i[iw];
+    v[2 * iw] = acc_r[iw];
+    v[2 * iw + 1] = acc_i[iw];
   }
   block_store<6>(v, rec);
   if (threadIdx.x >= 6 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
@@ -695,15 +831,15 @@ __device__ __forceinline__ void unocc_mixed_block(
   }
 }
 
-template <int K, bool N1LOG>
+template <int K, bool MASK, bool N1LOG, bool KAHAN>
 __global__ __launch_bounds__(256) void unocc_mixed_kernel(
     DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
     double *__restrict__ partials, double *__restrict__ n1log, float thresh,
     unsigned long long *__restrict__ fallback_count) {
   const int b = full_grid_block(p.NG, grp_lo, grp_cnt);
-  unocc_mixed_block<K, N1LOG>(p, n1_begin, n1_end, b,
-                              partials + (size_t)b * kSlots, n1log, thresh,
-                              fallback_count);
+  unocc_mixed_block<K, MASK, N1LOG, KAHAN>(p, n1_begin, n1_end, b,
+                                           partials + (size_t)b * kSlots,
+                                           n1log, thresh, fallback_count);
 }
 
 // ---------------------------------------------------------------------------
@@ -1602,8 +1738,8 @@ __global__ __launch_bounds__(256) __attribute__((
       unocc_fp64_block<K, SEM == kSemBgw, false>(p, p.NV, p.NB, w.block, rec,
                                                  nullptr);
     else
-      unocc_mixed_block<K, false>(p, p.NV, p.NB, w.block, rec, nullptr,
-                                  thresh, fallback_count);
+      unocc_mixed_block<K, SEM == kSemBgw, false, PREC == kPrecMixedKahan>(
+          p, p.NV, p.NB, w.block, rec, nullptr, thresh, fallback_count);
   } else {
     occ_block<SEM, false>(p, p.NV, w.block, rec, nullptr);
   }
@@ -1661,12 +1797,13 @@ hipError_t launch_gather_aqsm(const double2 *aqsm, const int *igp_of, int NB,
   return hipGetLastError();
 }
 
-hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, size_t n,
-                               hipStream_t s) {
+hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, int NB,
+                               int NC, hipStream_t s) {
+  const size_t n = (size_t)NB * padded_nc(NC);
   if (n == 0) return hipSuccess;
   size_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  to_fp32_kernel<<<(unsigned)blocks, 256, 0, s>>>(aqsn, out, n);
+  to_fp32_kernel<<<(unsigned)blocks, 256, 0, s>>>(aqsn, out, NB, NC);
   return hipGetLastError();
 }
 
@@ -1703,13 +1840,21 @@ static hipError_t launch_unocc_k(const DevProblem &p, int n1b, int n1e,
             <<<grid, blk, 0, s>>>(p, n1b, n1e, glo, gcnt, partials, n1log);
     }
   } else {
-    if (sem == kSemBgw) return hipErrorNotSupported;
-    if (lg)
-      unocc_mixed_kernel<K, true><<<grid, blk, 0, s>>>(
-          p, n1b, n1e, glo, gcnt, partials, n1log, (float)thresh, fb);
-    else
-      unocc_mixed_kernel<K, false><<<grid, blk, 0, s>>>(
-          p, n1b, n1e, glo, gcnt, partials, n1log, (float)thresh, fb);
+    const float th = (float)thresh;
+    const bool bgw = sem == kSemBgw;
+    const bool kahan = precision == kPrecMixedKahan;
+#define GPP_MIX(M, L, KA)                                                   \
+  unocc_mixed_kernel<K, M, L, KA><<<grid, blk, 0, s>>>(p, n1b, n1e, glo,    \
+                                                       gcnt, partials,      \
+                                                       n1log, th, fb)
+    if (kahan) {
+      if (bgw) { if (lg) GPP_MIX(true, true, true); else GPP_MIX(true, false, true); }
+      else { if (lg) GPP_MIX(false, true, true); else GPP_MIX(false, false, true); }
+    } else {
+      if (bgw) { if (lg) GPP_MIX(true, true, false); else GPP_MIX(true, false, false); }
+      else { if (lg) GPP_MIX(false, true, false); else GPP_MIX(false, false, false); }
+    }
+#undef GPP_MIX
   }
   return hipGetLastError();
 }
@@ -1934,7 +2079,6 @@ hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
                         hipStream_t s) {
   if (unocc_threads < 64 || unocc_threads > 256 || unocc_threads % 64)
     return hipErrorInvalidValue;
-  if (precision == 1 && sem == kSemBgw) return hipErrorNotSupported;
   batch_reset_kernel<<<1, 64, 0, s>>>(fallback_count);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -1942,8 +2086,20 @@ hipError_t launch_batch(const DevProblem *probs, const BatchItem *items_u,
   if (n_u > 0) {
 #define GPP_BL(KK)                                                          \
   case KK:                                                                  \
-    if (precision == 1)                                                     \
-      batch_kernel<KK, 1, kSemGppker, 0><<<n_u, unocc_threads, 0, s>>>(               \
+    if (precision != 0 && sem == kSemBgw) {                                 \
+      if (precision == kPrecMixedKahan)                                     \
+        batch_kernel<KK, kPrecMixedKahan, kSemBgw, 0>                       \
+            <<<n_u, unocc_threads, 0, s>>>(probs, items_u, partials, thr,   \
+                                           fallback_count);                 \
+      else                                                                  \
+        batch_kernel<KK, 1, kSemBgw, 0><<<n_u, unocc_threads, 0, s>>>(      \
+            probs, items_u, partials, thr, fallback_count);                 \
+    } else if (precision == kPrecMixedKahan)                                \
+      batch_kernel<KK, kPrecMixedKahan, kSemGppker, 0>                      \
+          <<<n_u, unocc_threads, 0, s>>>(probs, items_u, partials, thr,     \
+                                         fallback_count);                   \
+    else if (precision == 1)                                                \
+      batch_kernel<KK, 1, kSemGppker, 0><<<n_u, unocc_threads, 0, s>>>(     \
           probs, items_u, partials, thr, fallback_count);                   \
     else if (sem == kSemGppkerStrict)                                       \
       batch_kernel<KK, 0, kSemGppkerStrict, 0><<<n_u, unocc_threads, 0, s>>>(         \

@@ -18,6 +18,18 @@ constexpr double kOcc = 1.0;                     // occ
 // Semantics (mirrors GPP_SEM_* in include/gpp_sigma.h).
 enum Sem : int { kSemGppker = 0, kSemGppkerStrict = 1, kSemBgw = 2 };
 
+// The precision argument of launch_unocc / launch_batch: 0 FP64, 1 mixed,
+// kPrecMixedKahan mixed with a compensated FP32 run sum.
+constexpr int kPrecMixedKahan = 3;
+
+// Doubles per band of the prepass's frequency table DevProblem::wx.
+constexpr int kWxDoubles = 10;
+
+// Row length of the FP32 copy of aqsn (mixed precision): ncouls rounded up
+// to 32 G-vectors with zeros, so a lane run of up to 32 G-vectors is one
+// contiguous, in-bounds load.
+__host__ __device__ constexpr int padded_nc(int nc) { return (nc + 31) & ~31; }
+
 // Output slot layout of every partial-sum record (doubles):
 //   [0..5]  achtemp re/im for iw = 0,1,2
 //   [6..11] asxtemp re/im for iw = 0,1,2
@@ -32,8 +44,12 @@ struct DevProblem {
   const double *vcoul;    // [NC]
   const int *igp_of;      // [NG]: resolved igp in [0, NC) or -1 (skipped)
   const double *wx;       // [NB][3], clamped, band-resolved; then [NB][3]
-                          // wx^2 / |wx^2|^2 (occupied solver, gppker)
-  const float2 *aqsn32;   // [NB][NC] FP32 copy (mixed precision only)
+                          // wx^2 / |wx^2|^2 (occupied solver, gppker); then
+                          // [NB][8] floats: wx split into FP32 hi + lo per
+                          // frequency and 2 zeros (mixed precision).
+                          // kWxDoubles per band.
+  const float2 *aqsn32;   // [NB][padded_nc(NC)] FP32 copy, zero padded
+                          // (mixed precision only)
   const double2 *aqsmG;   // [NB][NG]: aqsm[n1][igp_of[my_igp]], 0 if skipped
   int NB, NV, NC, NG;
 };
@@ -66,8 +82,8 @@ hipError_t launch_prepass(const int *inv_igp_index, const int *indinv,
                           const double *wx_in, int wx_per_band, int NB,
                           int NC, int NG, int *igp_of, double *wx_out,
                           unsigned long long *counter, hipStream_t s);
-hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, size_t n,
-                               hipStream_t s);
+hipError_t launch_aqsn_to_fp32(const double2 *aqsn, float2 *out, int NB,
+                               int NC, hipStream_t s);
 hipError_t launch_gather_aqsm(const double2 *aqsm, const int *igp_of, int NB,
                               int NC, int NG, double2 *out, hipStream_t s);
 // Blocks of the non-occupied launch over all G' columns (= records).
@@ -113,7 +129,7 @@ int achstemp_records(const DevProblem &p, int sem);
 // Prepass, aqsm gather and (mixed) FP32 copy of aqsn for problems
 // [0, n) of a device table, one launch each (problem = blockIdx.y);
 // max_ng_nb3 = max over problems of max(NG, 3 NB), max_nbng / max_nbnc the
-// largest NB*NG / NB*NC.
+// largest NB*NG / NB*padded_nc(NC).
 hipError_t launch_prepare_batch(const DevProblem *probs, const PrepItem *preps,
                                 int n, int max_ng_nb3, size_t max_nbng,
                                 size_t max_nbnc, bool mixed,

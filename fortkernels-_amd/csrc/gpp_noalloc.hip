@@ -102,11 +102,11 @@ extern "C" int gpp_sigma_noalloc(const gpp_problem *p, const gpp_options *o,
   const bool run_occ = n1o_end > 0;
 
   // Layout of the scratch array, as gpp_sigma_ex lays out its handle
-  // buffers: [counter][igp_of NG][wx NB*6][aqsmG NB*NG][records][2 slots].
+  // buffers: [counter][igp_of NG][wx NB*9][aqsmG NB*NG][records][2 slots].
   gpp::Arena a;
   const size_t off_cnt = a.take(256);
   const size_t off_igp = a.take((size_t)NG * sizeof(int));
-  const size_t off_wx = a.take((size_t)NB * 6 * sizeof(double));
+  const size_t off_wx = a.take((size_t)NB * gpp::kWxDoubles * sizeof(double));
   const size_t off_aqsmG = a.take((size_t)NB * NG * 16);
   gpp::DevProblem dp{};
   dp.aqsm = (const double2 *)p->aqsmtemp;

@@ -53,9 +53,14 @@ typedef struct { double re, im; } gpp_c128;
 
 /* Arithmetic of the non-occupied (n1 >= nvband) sum. */
 #define GPP_PREC_FP64    0   /* complex128 throughout                      */
-#define GPP_PREC_MIXED   1   /* FP32 resolvent and products, FP64 band and
-                                cross-tile accumulation, FP64 for terms with
-                                |wx - wtilde|^2 below resonance_threshold  */
+#define GPP_PREC_MIXED   1   /* FP32 resolvent and products summed in FP32
+                                over each lane's run of tile_ig G-vectors,
+                                FP64 band and cross-tile accumulation; a run
+                                holding a term with |wx - wtilde|^2 below
+                                resonance_threshold (or, bgw, a term within
+                                4e-6 relative of the mask's threshold) is
+                                evaluated in FP64 instead, through a per-
+                                wave work list                            */
 #define GPP_PREC_FP64_FACTORED 2 /* FP64 with the band sum taken first:
                                 P[my_igp][ig] = sum_n1 conj(aqsm[n1][igp]) *
                                 aqsn[n1][ig] on the FP64 matrix cores, then
@@ -98,6 +103,9 @@ typedef struct { double re, im; } gpp_c128;
 #define GPP_FLAG_NO_ACHSTEMP      8  /* skip the static remainder achstemp */
 #define GPP_FLAG_AUTOTUNE        16  /* tune the tile shape for this shape
                                         class first if it is not cached    */
+#define GPP_FLAG_MIXED_KAHAN     32  /* GPP_PREC_MIXED: Kahan-compensated
+                                        FP32 run sums (4 more FP32
+                                        operations per term)               */
 
 /* One sigma evaluation.  Shapes and arrays as in gppKer.cpp:157-264. */
 typedef struct {
@@ -126,8 +134,13 @@ typedef struct {
   int flags;                      /* GPP_FLAG_*                            */
   void *stream;                   /* hipStream_t for GPP_MEM_DEVICE work;
                                      NULL = the handle's own stream        */
-  double resonance_threshold;     /* mixed: FP64 for |wdiff|^2 below this;
-                                     <= 0 selects the default 1e-6         */
+  double resonance_threshold;     /* mixed: a lane run with a term of
+                                     |wdiff|^2 below this is evaluated in
+                                     FP64; <= 0 selects the default 1e-6
+                                     (FP32 underflow and exact resonances);
+                                     larger values trade speed for
+                                     accuracy where near-resonant terms
+                                     dominate the sum                     */
   int tile_ig;                    /* G-vectors per lane run (0 = tuned)    */
   int waves_per_block;            /* 0 = tuned                             */
 } gpp_options;

@@ -50,6 +50,28 @@ MIXED_LARGE_BOUND = 5e-8
 MIXED_BATCH_BOUND = 1e-6
 
 
+@pytest.mark.parametrize("cfg,bound", [(2, 5e-7), (3, 4e-8)])
+def test_canonical_shape_mixed_full(handle, cfg, bound):
+    """512/2/32768/20 in full, mixed against FP64 per entry (measured
+    2.3e-7 with the canonical distributions, 1.9e-8 band-resolved); on the
+    band-resolved inputs a resonance window of 4e-3 brings every entry below
+    1e-8 (measured 5.5e-9)."""
+    import torch
+    dev = gppk.fill_inputs_device(cfg, 512, 32768, 20, nvband=2, seed=2024)
+    full = handle.sigma(dev)
+    mixed = handle.sigma(dev, precision="mixed")
+    e = entry_err(mixed, full)
+    say(f"canonical shape config {cfg}: mixed vs fp64 per entry {e:.3e}")
+    assert e <= bound
+    if cfg == 3:
+        win = handle.sigma(dev, precision="mixed", resonance_threshold=4e-3)
+        e = entry_err(win, full)
+        say(f"canonical shape config 3, window 4e-3: {e:.3e}")
+        assert e <= 1e-8
+    del dev
+    torch.cuda.empty_cache()
+
+
 def test_large_instance_2048_16_65536_20(handle):
     import torch
     say("large: generating 2048/16/65536/20 on the device")

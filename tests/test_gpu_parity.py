@@ -148,22 +148,24 @@ def test_invalid_arguments_are_errors(handle):
         handle.sigma(inp, tile_ig=5)
     assert ei.value.code == gppk.GPP_EINVAL
     with pytest.raises(gppk.GppError) as ei:
-        handle.sigma(inp, precision="mixed", semantics="bgw")
-    assert ei.value.code == gppk.GPP_EUNSUPPORTED
+        handle.sigma(inp, precision=7)
+    assert ei.value.code == gppk.GPP_EINVAL
 
 
 def test_mixed_precision_error(handle):
     say("mixed")
-    for cfg in (2, 3):
+    # per-entry bounds about 2.5x the errors measured on these inputs
+    # (1.1e-7 and 1.6e-8); test_gpu_mixed.py holds the rest of the mixed path
+    for cfg, bound in ((2, 3e-7), (3, 4e-8)):
         inp = gppk.fill_inputs(cfg, 16, 2048, 8, nvband=2, seed=8)
         ref = orc.sigma(inp)
-        # a wide FP64 window on config 3 so that the redo path is exercised
+        # a wide FP64 window on config 3 so that the work list is exercised
         got = handle.sigma(inp, precision="mixed",
                            resonance_threshold=1e-2 if cfg == 3 else 0.0)
         e = rel_err(got["achtemp"], ref["achtemp"])
         say(f"config {cfg} mixed achtemp rel err {e:.3e} fallback terms "
             f"{got['fp64_fallback_terms']}")
-        assert e < 2e-6
+        assert e < bound
         # occupied bands always run in FP64
         assert rel_err(got["asxtemp"], ref["asxtemp"]) <= FP64_TOL
         if cfg == 3:
@@ -285,7 +287,7 @@ def test_acht_n1_loc_mixed(handle):
     inp = gppk.fill_inputs(2, 6, 512, 8, nvband=2, seed=22)
     got = handle.sigma(inp, precision="mixed", acht_n1_loc=True)
     ref = orc.sigma(inp, want_acht_n1=True)
-    assert rel_err(got["acht_n1_loc"], ref["acht_n1_loc"]) < 2e-6
+    assert rel_err(got["acht_n1_loc"], ref["acht_n1_loc"]) < 1e-6
 
 
 def _batch_inputs(seed=31, n=7):
@@ -320,7 +322,7 @@ def test_batch_mixed(handle):
     res = handle.batch(inputs, precision="mixed")
     for got, d in zip(res, inputs):
         ref = orc.sigma(d)
-        assert rel_err(got["achtemp"], ref["achtemp"]) < 2e-6
+        assert rel_err(got["achtemp"], ref["achtemp"]) < 1e-6
         assert rel_err(got["asxtemp"], ref["asxtemp"]) <= FP64_TOL
 
 

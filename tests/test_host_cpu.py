@@ -128,14 +128,15 @@ def test_noalloc_refuses_host_memory_mixed_and_flags():
 
 def _scratch_bytes(nb, nc, ng, nv, k=8, w=4, factored=False):
     """The entry point's scratch layout, recomputed: 256-byte slices for the
-    counter, igp table, wx table, gathered aqsm and the block records."""
+    counter, igp table, wx table (10 doubles per band), gathered aqsm and the
+    block records."""
     def up(b):
         return (b + 255) // 256 * 256
     ngrp = (ng + 63) // 64
     n_un = ngrp * ((nc + 63) // 64) if factored else ngrp * ((nc + k * w - 1) // (k * w))
     n_occ = ng * ((nc + 2047) // 2048) if nv > 0 else 0
     n_ach = ng
-    return (up(256) + up(ng * 4) + up(nb * 48) + up(nb * ng * 16)
+    return (up(256) + up(ng * 4) + up(nb * 80) + up(nb * ng * 16)
             + up((n_un + n_occ + n_ach) * 96) + up(2 * 96))
 
 
