@@ -67,9 +67,23 @@ int set_err(gpp_handle h, int code, const char *fmt, ...) {
   return code;
 }
 
+// Failure injection for tests: with GPP_FAIL_ALLOC_AFTER=N in the
+// environment, the N-th buffer growth of a handle (device buffers and the
+// pinned result buffer, counted from 1 per handle) fails as an
+// out-of-memory error would, before anything is freed or allocated.
+bool simulated_alloc_failure(gpp_handle h) {
+  const char *v = std::getenv("GPP_FAIL_ALLOC_AFTER");
+  const long long n = ++h->growths;
+  return v && *v && n == std::atoll(v);
+}
+
 int ensure(gpp_handle h, DevBuf &b, size_t bytes) {
   if (bytes == 0) bytes = 256;
   if (b.cap >= bytes) return GPP_OK;
+  if (simulated_alloc_failure(h))
+    return set_err(h, GPP_ENOMEM,
+                   "device buffer of %zu bytes: simulated allocation failure "
+                   "(GPP_FAIL_ALLOC_AFTER)", bytes);
   if (b.ptr) {
     HIP_OR_RET(h, hipStreamSynchronize(h->stream));
     HIP_OR_RET(h, hipDeviceSynchronize());
@@ -87,6 +101,10 @@ int ensure(gpp_handle h, DevBuf &b, size_t bytes) {
 
 int ensure_host_out(gpp_handle h, size_t doubles) {
   if (h->host_out_cap >= doubles) return GPP_OK;
+  if (simulated_alloc_failure(h))
+    return set_err(h, GPP_ENOMEM,
+                   "pinned result buffer: simulated allocation failure "
+                   "(GPP_FAIL_ALLOC_AFTER)");
   if (h->host_out) (void)hipHostFree(h->host_out);
   h->host_out = nullptr;
   h->host_out_cap = 0;

@@ -857,8 +857,10 @@ constexpr int kOccChunk = 2048;  // G-vectors per block
 // The band-independent part of one (my_igp, ig) of the occupied solver.
 struct OccG {
   double wr, wi, wy2;  // wtilde, wtilde.im^2
+  double wiwr;         // wtilde.im * wtilde.re
   double w2r, w2i;     // wtilde2 = wtilde^2 (gppKer.cpp:132)
   double omr, omi;     // Omega2 = wtilde2 * I_eps (gppKer.cpp:133)
+  double er, ei;       // I_eps[my_igp][ig], Omega2's entry
   double exr, exi;     // the I_eps entry used for sch and the cutoff
   double cut2;         // (sexcut * |I_eps|)^2
 };
@@ -869,10 +871,13 @@ __device__ __forceinline__ OccG occ_g_of(double2 w, double2 e, double2 ex) {
   g.wr = w.x;
   g.wi = w.y;
   g.wy2 = w.y * w.y;
+  g.wiwr = w.y * w.x;
   g.w2r = w.x * w.x - g.wy2;
   g.w2i = 2.0 * w.x * w.y;
   g.omr = g.w2r * e.x - g.w2i * e.y;
   g.omi = g.w2r * e.y + g.w2i * e.x;
+  g.er = e.x;
+  g.ei = e.y;
   g.exr = ex.x;
   g.exi = ex.y;
   g.cut2 = kSexcut * kSexcut * (ex.x * ex.x + ex.y * ex.y);
@@ -910,12 +915,24 @@ __device__ __forceinline__ OccBand occ_band(const Prob &p, int n1) {
 }
 
 // One (n1, my_igp, ig), three frequencies: adds matngmatmgp * sch to ts and
-// matngmatmgp * ssx to tx (gppKer.cpp:27-61, 141-142).
+// matngmatmgp * ssx to tx (gppKer.cpp:27-61, 141-142; f90:290-329).
+//   first branch  (|wdiff|^2 > limittwo, |delw|^2 < limitone):
+//     sch = delw * I_eps_x, ssx = Omega2 * conj(cden) / |cden|^2 with
+//     cden = wxt^2 (gppker) or wxt^2 - wtilde2 (bgw);
+//   second branch (|delw|^2 > to1): sch = 0 and, with cden =
+//     4 wtilde2 (delw + 1/2), ssx = -Omega2 conj(cden) delw / |cden|^2 =
+//     -Omega2 delw / cden = -I_eps delw / (4 delw + 2): wtilde2 cancels
+//     (it is nonzero there: delw = 0 when wtilde = 0), which spares the
+//     product with wtilde2 and Omega2;
+//   else 0.
+// The products with matngmatmgp m are formed once per (n1, ig) for the three
+// frequencies (m * I_eps_x for sch, m * Omega2 for the gppker ssx).
 template <int SEM>
 __device__ __forceinline__ void occ_term(const OccG &g, const OccBand &B,
                                          double mr, double mi,
                                          double (&tsr)[3], double (&tsi)[3],
                                          double (&txr)[3], double (&txi)[3]) {
+  constexpr bool BGW = SEM == kSemBgw;
   // wdiff = wxt - wtilde = (dr, -wi); conj(wdiff) = (dr, wi)
   double dr[3], wdiffr[3], rden[3];
 #pragma unroll
@@ -924,51 +941,64 @@ __device__ __forceinline__ void occ_term(const OccG &g, const OccBand &B,
     wdiffr[iw] = fma(dr[iw], dr[iw], g.wy2);
   }
   recip3(wdiffr[0], wdiffr[1], wdiffr[2], rden[0], rden[1], rden[2]);
+  // bgw first branch: 1 / |wxt^2 - wtilde2|^2 for the three frequencies
+  double cdr[3], rcd[3];
+  if (BGW) {
+    double cmag[3];
+#pragma unroll
+    for (int iw = 0; iw < 3; ++iw) {
+      cdr[iw] = fma(B.wxt[iw], B.wxt[iw], -g.w2r);
+      cmag[iw] = fma(cdr[iw], cdr[iw], g.w2i * g.w2i);
+    }
+    recip3(cmag[0], cmag[1], cmag[2], rcd[0], rcd[1], rcd[2]);
+  }
+  const double mxr = mr * g.exr - mi * g.exi;  // m * I_eps_x
+  const double mxi = mr * g.exi + mi * g.exr;
+  const double mor = mr * g.omr - mi * g.omi;  // m * Omega2
+  const double moi = mr * g.omi + mi * g.omr;
 #pragma unroll
   for (int iw = 0; iw < 3; ++iw) {
     // delw = wtilde * conj(wdiff) * rden
-    const double dlr = (g.wr * dr[iw] - g.wy2) * rden[iw];
-    const double dli = (g.wr * g.wi + g.wi * dr[iw]) * rden[iw];
-    const double delwr = dlr * dlr + dli * dli;
-    double schr = 0.0, schi = 0.0, ssr = 0.0, ssi = 0.0;
+    const double dlr = fma(g.wr, dr[iw], -g.wy2) * rden[iw];
+    const double dli = fma(g.wi, dr[iw], g.wiwr) * rden[iw];
+    const double delwr = fma(dlr, dlr, dli * dli);
+    double ssr = 0.0, ssi = 0.0;  // ssx (bgw, and the second branch)
+    bool have_ssx = false;
     if (wdiffr[iw] > kLimitTwo && delwr < kLimitOne) {
-      schr = dlr * g.exr - dli * g.exi;
-      schi = dlr * g.exi + dli * g.exr;
-      if (SEM == kSemBgw) {
-        // cden = wxt^2 - wtilde2 (f90:303)
-        const double cr = B.wxt[iw] * B.wxt[iw] - g.w2r;
+      tsr[iw] = fma(dlr, mxr, fma(-dli, mxi, tsr[iw]));
+      tsi[iw] = fma(dlr, mxi, fma(dli, mxr, tsi[iw]));
+      if (BGW) {
+        // cden = wxt^2 - wtilde2 (f90:303): Omega2 * conj(cden) / |cden|^2
         const double ci = -g.w2i;
-        const double rd = recip(cr * cr + ci * ci);
-        ssr = (g.omr * cr + g.omi * ci) * rd;
-        ssi = (g.omi * cr - g.omr * ci) * rd;
+        ssr = fma(g.omr, cdr[iw], g.omi * ci) * rcd[iw];
+        ssi = fma(g.omi, cdr[iw], -g.omr * ci) * rcd[iw];
+        have_ssx = true;
       } else {
-        ssr = g.omr * B.fssx[iw];
-        ssi = g.omi * B.fssx[iw];
+        txr[iw] = fma(mor, B.fssx[iw], txr[iw]);
+        txi[iw] = fma(moi, B.fssx[iw], txi[iw]);
       }
     } else if (delwr > kTo1) {
-      // cden = 4 * wtilde2 * (delw + 0.5)
-      const double ar = dlr + 0.5, ai = dli;
-      const double cr = 4.0 * (g.w2r * ar - g.w2i * ai);
-      const double ci = 4.0 * (g.w2r * ai + g.w2i * ar);
-      const double rd = recip(cr * cr + ci * ci);
-      // ssx = -Omega2 * conj(cden) * rden * delw
-      const double tr = -(g.omr * cr + g.omi * ci) * rd;
-      const double ti = -(g.omi * cr - g.omr * ci) * rd;
-      ssr = tr * dlr - ti * dli;
-      ssi = tr * dli + ti * dlr;
+      // q = 4 delw + 2; ssx = -I_eps * delw * conj(q) / |q|^2
+      const double qr = fma(4.0, dlr, 2.0), qi = 4.0 * dli;
+      const double rq = recip(fma(qr, qr, qi * qi));
+      const double nr = g.er * dlr - g.ei * dli;  // I_eps * delw
+      const double ni = g.er * dli + g.ei * dlr;
+      ssr = -fma(nr, qr, ni * qi) * rq;
+      ssi = -fma(ni, qr, -nr * qi) * rq;
+      have_ssx = true;
     }
-    if (SEM == kSemBgw) {
-      // f90:321: |ssx| > sexcut*|I_eps| and wxt < 0 -> ssx = 0
-      if (B.wxt[iw] < 0.0 && (ssr * ssr + ssi * ssi) > g.cut2) {
-        ssr = 0.0;
-        ssi = 0.0;
+    if (have_ssx) {
+      if (BGW) {
+        // f90:321: |ssx| > sexcut*|I_eps| and wxt < 0 -> ssx = 0
+        if (B.wxt[iw] < 0.0 && fma(ssr, ssr, ssi * ssi) > g.cut2) {
+          ssr = 0.0;
+          ssi = 0.0;
+        }
       }
+      // (gppKer.cpp:58 tests abs(wxt) < 0, which never holds.)
+      txr[iw] = fma(mr, ssr, fma(-mi, ssi, txr[iw]));
+      txi[iw] = fma(mr, ssi, fma(mi, ssr, txi[iw]));
     }
-    // (gppKer.cpp:58 tests abs(wxt) < 0, which never holds.)
-    txr[iw] += mr * ssr - mi * ssi;
-    txi[iw] += mr * ssi + mi * ssr;
-    tsr[iw] += mr * schr - mi * schi;
-    tsi[iw] += mr * schi + mi * schr;
   }
 }
 
@@ -1034,8 +1064,24 @@ __device__ __forceinline__ void occ_block(const Prob &p, int n1_end,
       occ_term<SEM>(g, B, mr, mi, tsr, tsi, txr, txi);
     }
   } else {
+    // wtilde and I_eps of the thread's next G-vector are loaded before the
+    // current one's bands are evaluated, so that with few occupied bands
+    // (2 at the canonical shape) the loads overlap the solver.
+    const size_t row = (size_t)my_igp * NC;
+    const size_t rowx = (SEM == kSemGppkerStrict) ? (size_t)my_igp * p.NG : row;
+    double2 w_nxt = make_double2(0.0, 0.0), e_nxt = w_nxt, x_nxt = w_nxt;
+    if (ig_begin < ig_end) {
+      w_nxt = ld(p.wtilde + row + ig_begin);
+      e_nxt = ld(p.ieps + row + ig_begin);
+      if (SEM == kSemGppkerStrict) x_nxt = ld(p.ieps + rowx + ig_begin);
+    }
     for (int ig = ig_begin; ig < ig_end; ig += blockDim.x) {
-      const OccG g = occ_g<SEM>(p, my_igp, ig);
+      const OccG g = occ_g_of(w_nxt, e_nxt,
+                              SEM == kSemGppkerStrict ? x_nxt : e_nxt);
+      const int ign = min(ig + (int)blockDim.x, ig_end - 1);
+      w_nxt = ld(p.wtilde + row + ign);
+      e_nxt = ld(p.ieps + row + ign);
+      if (SEM == kSemGppkerStrict) x_nxt = ld(p.ieps + rowx + ign);
       for (int n1 = 0; n1 < n1_end; ++n1) {
         const double2 am = ld(p.aqsm + (size_t)n1 * NC + igp);
         const OccBand B = occ_band(p, n1);

@@ -112,4 +112,5 @@ struct gpp_handle_s {
   int batch_nrec = 0;
   std::string err;
   gpp::Stager *stager = nullptr;  // host-input copies (created on first use)
+  long long growths = 0;          // buffer growths (failure injection count)
 };

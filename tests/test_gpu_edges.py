@@ -180,3 +180,21 @@ def test_dropin_from_two_threads():
         for rc, ach, asx in results[i]:
             assert rc == 0
             check({"achtemp": ach, "asxtemp": asx}, refs[i])
+
+
+@pytest.mark.parametrize("sem", ["gppker", "strict", "bgw"])
+def test_second_branch_next_to_cutoff(handle, sem):
+    """The occupied solver's second branch, evaluated on the GPU as
+    -I_eps delw / (4 delw + 2), against the oracle's form of the reference
+    (-Omega2 conj(cden) delw / |cden|^2) and the closed form, with |ssx|
+    from 2 to 10 |I_eps| around the bgw cutoff (4 |I_eps|, wx < 0)."""
+    from test_oracle_cpu import branch2_expected, branch2_problem
+    inp = branch2_problem()
+    ref = orc.sigma(inp, sem)
+    for prec in ("fp64", "mixed"):
+        got = handle.sigma(inp, precision=prec, semantics=sem)
+        check(got, ref, keys=("asxtemp",))
+        assert rel_err(got["asxtemp"], branch2_expected(
+            "bgw" if sem == "bgw" else "gppker")) <= 1e-12
+    for r in handle.batch([inp, inp], semantics=sem):
+        check(r, ref, keys=("asxtemp",))

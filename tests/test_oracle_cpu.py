@@ -643,3 +643,64 @@ def test_oracle_bgw_mask_second_condition():
     assert rel_err(got, want) < 1e-12, (got, want)
     keep, _ = masked_expected(inp, placed, "gppker")
     assert rel_err(keep, want) > 1e-3
+
+
+# The occupied solver's second branch next to the bgw cutoff.
+BRANCH2_EPS = (0.01, 0.02, 0.03, 0.05)
+BRANCH2_WX = np.array([[-0.2, 3.0, -5.0]])
+
+
+def branch2_problem():
+    """One occupied band, constant fills c = 0.5+0.5i except wtilde =
+    0.2 + eps i at four entries, band-resolved wx = (-0.2, 3, -5).  At
+    wx = -0.2 those four take the solver's second branch (|wdiff|^2 =
+    0.16 + eps^2 <= 0.25) with delw close to -1/2, where cden = 4 wtilde2
+    (delw + 1/2) is small and |ssx| = |I_eps delw / (4 delw + 2)| runs from
+    10 |I_eps| (eps = 0.01, cut by bgw: wx < 0) to 2 |I_eps| (0.05, kept);
+    every other term takes the first branch."""
+    inp = gppk.fill_inputs(1, 1, 64, 8, nvband=1)
+    w = inp["wtilde_array"].copy()
+    for k, eps in enumerate(BRANCH2_EPS):
+        w[k, 3 + 7 * k] = 0.2 + eps * 1j
+    return dict(inp, wtilde_array=w, wx_array=BRANCH2_WX.copy())
+
+
+def branch2_expected(sem):
+    """asxtemp from the solver written out per term, the second branch as
+    -I_eps delw / (4 delw + 2) (the reference's -Omega2 conj(cden) delw /
+    |cden|^2 with cden = 4 wtilde2 (delw + 1/2), wtilde2 cancelled)."""
+    inp = branch2_problem()
+    ng, nc = inp["I_eps_array"].shape
+    w = inp["wtilde_array"]
+    out = np.zeros(3, complex)
+    for iw, wx in enumerate(BRANCH2_WX[0]):
+        wd = wx - w
+        delw = w / wd
+        b1 = (np.abs(wd) ** 2 > 0.25) & (np.abs(delw) ** 2 < 2.5e5)
+        if sem == "bgw":
+            ssx1 = w ** 2 * C * np.conj(wx * wx - w ** 2) / np.abs(wx * wx - w ** 2) ** 2
+        else:
+            ssx1 = w ** 2 * C / wx ** 2
+        ssx2 = -C * delw / (4 * delw + 2)
+        ssx = np.where(b1, ssx1, np.where(np.abs(delw) ** 2 > 1e-6, ssx2, 0))
+        if sem == "bgw" and wx < 0:
+            ssx = np.where(np.abs(ssx) > 4 * abs(C), 0, ssx)
+        out[iw] = np.sum(ssx) * abs(C) ** 2 * (-1 if sem == "bgw" else 1)
+    return out
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_oracle_second_branch_next_to_cutoff(sem):
+    inp = branch2_problem()
+    got = orc.sigma(inp, sem)["asxtemp"]
+    want = branch2_expected(sem)
+    assert rel_err(got, want) <= 1e-12, (got, want)
+    # the four entries really are in the second branch, two of them past
+    # the cutoff
+    w = inp["wtilde_array"]
+    wd = -0.2 - w
+    delw = w / wd
+    sel = np.abs(wd) ** 2 <= 0.25
+    assert sel.sum() == 4
+    r = np.abs(delw / (4 * delw + 2))[sel]
+    assert (r > 4).sum() == 2 and (r < 4).sum() == 2
