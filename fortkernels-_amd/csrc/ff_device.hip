@@ -15,7 +15,7 @@
 // NG*NC complex numbers read once per pass for up to 8 bands: the pass is
 // HBM-bound (4 FMA per band per 16 bytes).
 //
-// The frequency-grid terms (achDtemp_Kernel :155, ffKernel.f90:335-384)
+// The frequency-grid terms (achDtemp_Kernel :155, ffKernel.f90:348-411)
 // touch no large array: one thread per (n1, iw) runs the grid.
 #include <algorithm>
 
@@ -218,16 +218,9 @@ __global__ __launch_bounds__(kRedThreads) void slice_reduce_kernel(
   if (lane < kRecSlots) T[(size_t)job * kRecSlots + slot] = v;
 }
 
-// |x| clamped to [1e-4, 1e4] as ffKernel.f90:369-371 (NaN passes through).
-__device__ __forceinline__ double clamp_ratio(double x) {
-  if (x < 1e-4) x = 1e-4;
-  if (x > 1e4) x = 1e4;
-  return x;
-}
-
 // The per-(n1, f) values of the grid terms that do not depend on iw, formed
 // once (one thread each) with the operations of the reference's loop body
-// (ffKernel.f90:335-384, ffKernel.cpp:200-206); row layout:
+// (ffKernel.f90:363-382, ffKernel.cpp:200-206); row layout:
 //   0,1  i pref[f] schDt[n1][f]        2  zb - Re brd        3  cor shift
 //   4    Im brd      5  right = zb     6  left = zb[f-1]     7  (Im brd)^2
 //   8,9  i pref_zb avg (sch2 factor)   10,11 schDt[f-1]
@@ -289,7 +282,7 @@ __global__ void freq_prep_kernel(FreqArgs a, int sem,
 // One thread per (n1 = blockIdx.y, iw).  SEM 0 (ffKernel.cpp:200-206):
 //   ach -= (i pref[f] schDt[n1][f]) / (wx - coh[f]),  every f,
 //   wx = freqevalmin - ekq[n1] + (iw-1)*freqevalstep.
-// SEM 1 (ffKernel.f90:335-384), wx = freqevalmin - ekq[n1] + iw*freqevalstep:
+// SEM 1 (ffKernel.f90:361-409), wx = freqevalmin - ekq[n1] + iw*freqevalstep:
 //   f != nFreq-1: ach  -= i pref schDt / (wx - coh); corb -= ... / (wx - cor)
 //   f != 0:       sch2 -= i pref_zb avg * (-log|ratio|) ; cor -= i pref_zb lin3
 // A band's rows are wave-uniform (scalar loads).  blockIdx.z picks a
@@ -305,6 +298,13 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
                                                    double *__restrict__ part) {
   const int n1 = OCC ? (int)blockIdx.y : a.NV + (int)blockIdx.y;
   const int iw = blockIdx.x * blockDim.x + threadIdx.x;
+  // the logarithm's table (4 KiB), read per lane from LDS
+  __shared__ double logtab[SEM == 1 ? 2 << kLogTableBits : 1];
+  if constexpr (SEM == 1) {
+    for (int i = threadIdx.x; i < (2 << kLogTableBits); i += blockDim.x)
+      logtab[i] = kLogTable[i];
+    __syncthreads();
+  }
   if (iw >= a.NE) return;
   const int f0 = blockIdx.z * fchunk;
   const int f1 = min(a.nFreq, f0 + fchunk);
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
   const double *rb = rows + (size_t)n1 * a.nFreq * kRowDoubles;
   double achr = 0.0, achi = 0.0, cbr = 0.0, cbi = 0.0;
   double s2r = 0.0, s2i = 0.0, cr = 0.0, ci = 0.0;
-  // occupied lin3 argument: -freqevalmin - ekq + (iw-1)*freqevalstep (f90:375)
+  // occupied lin3 argument: -freqevalmin - ekq + (iw-1)*freqevalstep (f90:403)
   const double wocc = -a.fmin - ek + iw * a.fstep;
   int f = f0;
   if constexpr (SEM == 0) {
@@ -332,41 +332,83 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
       achi -= fma(rn[1], xb, -rn[5]) * qb;
     }
   }
-  for (; f < f1; ++f) {
-    const double *r = rb + (size_t)f * kRowDoubles;
-    const double2 num = make_double2(r[0], r[1]);
-    if (SEM == 0 || f != a.nFreq - 1) {
+  if constexpr (SEM == 0) {
+    for (; f < f1; ++f) {
+      const double *r = rb + (size_t)f * kRowDoubles;
+      const double2 t1 = cdiv_y2(make_double2(r[0], r[1]),
+                                 make_double2(wx - r[2], r[4]), r[7]);
+      achr -= t1.x;
+      achi -= t1.y;
+    }
+  } else {
+    // Pole terms, f != nFreq-1 (f90:386-387); two grid points share a
+    // reciprocal.  Unoccupied bands: the cor pole is the coh pole (row 3 ==
+    // row 2), so corb equals ach and is copied after the loops.
+    const int fp1 = min(f1, a.nFreq - 1);
+    for (; f + 1 < fp1; f += 2) {
+      const double *r = rb + (size_t)f * kRowDoubles;
+      const double *rn = r + kRowDoubles;
+      const double xa = wx - r[2], xb = wx - rn[2];
+      double qa, qb;
+      recip2(fma(xa, xa, r[7]), fma(xb, xb, rn[7]), qa, qb);
+      // -(i pref schDt) / (x + i brd.im) = -num conj(x + i brd.im) q
+      achr -= fma(r[0], xa, r[1] * r[4]) * qa;
+      achi -= fma(r[1], xa, -r[0] * r[4]) * qa;
+      achr -= fma(rn[0], xb, rn[1] * rn[4]) * qb;
+      achi -= fma(rn[1], xb, -rn[0] * rn[4]) * qb;
+      if (OCC) {
+        const double ya = wx - r[3], yb = wx - rn[3];
+        recip2(fma(ya, ya, r[7]), fma(yb, yb, rn[7]), qa, qb);
+        cbr -= fma(r[0], ya, r[1] * r[4]) * qa;
+        cbi -= fma(r[1], ya, -r[0] * r[4]) * qa;
+        cbr -= fma(rn[0], yb, rn[1] * rn[4]) * qb;
+        cbi -= fma(rn[1], yb, -rn[0] * rn[4]) * qb;
+      }
+    }
+    for (; f < fp1; ++f) {
+      const double *r = rb + (size_t)f * kRowDoubles;
+      const double2 num = make_double2(r[0], r[1]);
       const double2 t1 = cdiv_y2(num, make_double2(wx - r[2], r[4]), r[7]);
       achr -= t1.x;
       achi -= t1.y;
-      if (SEM == 1) {
-        // unoccupied bands: the cor pole is the coh pole (row 3 == row 2),
-        // so corb equals ach and is copied after the loop
-        if (OCC) {
-          const double2 t2 = cdiv_y2(num, make_double2(wx - r[3], r[4]), r[7]);
-          cbr -= t2.x;
-          cbi -= t2.y;
-        }
+      if (OCC) {
+        const double2 t2 = cdiv_y2(num, make_double2(wx - r[3], r[4]), r[7]);
+        cbr -= t2.x;
+        cbi -= t2.y;
       }
     }
-    if (SEM == 1 && f != 0) {
-      const double right = r[5], left = r[6];
-      double it = -log_pos(clamp_ratio(fabs((wx - right) / (wx - left))));
-      // sch2 -= i pref_zb * avg * it
-      s2r -= r[8] * it;
-      s2i -= r[9] * it;
-      double ar;
-      if (occ) {
-        it = log_pos(clamp_ratio(fabs((wx + right) / (wx + left))));
-        ar = wocc - left;
-      } else {
-        ar = wx - left;
+    // Interval terms, f != 0 (f90:391-409): log|(wx - right)/(wx - left)|
+    // as the difference of log|wx - zb| at the interval's ends, carried from
+    // one grid point to the next (right = zb[f] = row 5, left = zb[f-1]).
+    const int fi0 = max(f0, 1);
+    if (fi0 < f1) {
+      const double zl = rb[(size_t)(fi0 - 1) * kRowDoubles + 5];
+      double lw = log_abs(wx - zl, logtab);              // log|wx - left|
+      double lo = OCC ? log_abs(wx + zl, logtab) : 0.0;  // log|wx + left|
+      for (f = fi0; f < f1; ++f) {
+        const double *r = rb + (size_t)f * kRowDoubles;
+        const double right = r[5], left = r[6];
+        const double lr = log_abs(wx - right, logtab);
+        double it = -clamp_log_ratio(lr - lw);
+        lw = lr;
+        // sch2 -= i pref_zb * avg * it
+        s2r -= r[8] * it;
+        s2i -= r[9] * it;
+        double ar;
+        if (OCC) {
+          const double lro = log_abs(wx + right, logtab);
+          it = clamp_log_ratio(lro - lo);
+          lo = lro;
+          ar = wocc - left;
+        } else {
+          ar = wx - left;
+        }
+        const double l3r = (r[10] + r[12] * ar) * it + r[14];
+        const double l3i = (r[11] + r[13] * ar) * it + r[15];
+        // cor -= i pref_zb * lin3
+        cr -= -a.pref_zb * l3i;
+        ci -= a.pref_zb * l3r;
       }
-      const double l3r = (r[10] + r[12] * ar) * it + r[14];
-      const double l3i = (r[11] + r[13] * ar) * it + r[15];
-      // cor -= i pref_zb * lin3
-      cr -= -a.pref_zb * l3i;
-      ci -= a.pref_zb * l3r;
     }
   }
   if (SEM == 1 && !OCC) {

@@ -1,25 +1,28 @@
 // ff_math.h — natural logarithm for the FF frequency-grid kernel.
 //
-// The grid terms of the FF Coulomb-hole sums take log|ratio| with the ratio
-// clamped to [1e-4, 1e4].  The logarithm of the device math library spends
-// about 100 FP64 instructions on double-double corrections; this one uses
-// about 25, at most 1.3 ulp from the exact value (0.5 ulp near 1), checked
-// against long double logl over the clamped range.
+// The interval terms of ffKernel.f90 need log|(wx - right)/(wx - left)| for
+// consecutive grid points, clamped to [log 1e-4, log 1e4]; the kernel forms
+// it as the difference of log|wx - zb[f]| at adjacent points (one logarithm
+// per point, no division) and clamps the difference.  The logarithm of the
+// device math library spends about 100 FP64 instructions on double-double
+// corrections; log_abs below takes 17 and one table read (from LDS on the
+// device), with an absolute error below 1.5 ulp of max(|log x|, ln 2),
+// which is what a difference of two logarithms needs.
 //
-// Method (the classical reduction, as in fdlibm's e_log.c): x = 2^k (1+f)
-// with sqrt(2)/2 <= 1+f < sqrt(2), s = f/(2+f), z = s^2 <= (3 - 2 sqrt 2)^2,
-//   log(1+f) = 2 atanh(s) = f - f^2/2 + s (f^2/2 + R),
-//   R = z P(z),  P(z) = sum_{j>=0} 2 z^j / (2j+3),
-// log x = k ln2_hi + (log(1+f) + k ln2_lo), ln2_hi having 32 significant
-// bits so that k ln2_hi is exact.  P is replaced by the degree-6 polynomial
-// Q obtained by Chebyshev economization of the series on [0, 0.0295]
-// (computed in exact rational arithmetic, then rounded to double):
-// |P - Q| <= 3.2e-16 there, below 5e-18 of the logarithm.
+// Method (table-driven, Tang, ACM TOMS 16(4), 1990): |x| = m 2^k with m in
+// [0.5, 1) (frexp, exact for subnormals too); the top 8 fraction bits of m
+// pick {invc, logc} (ff_log_table.h), r = m invc - 1 (one fma, |r| < 2^-9),
+//   log|x| = (k ln2_hi + logc) + (k ln2_lo + r + r^2 Q(r)),
+// ln2_hi having 32 significant bits so that k ln2_hi is exact, Q the cubic
+// of ff_log_table.h.
 //
 // Compiled for the device (hipcc) and for the host (g++, to check it).
 #pragma once
 
 #include <cmath>
+#include <cstdint>
+
+#include "ff_log_table.h"
 
 #if defined(__HIPCC__)
 #define FF_HD __host__ __device__
@@ -30,43 +33,42 @@
 namespace gpp {
 namespace ff {
 
-// 1/d for d in [1.5, 2.5]: the hardware reciprocal refined by one Newton
-// step on the device, the IEEE quotient on the host.
-FF_HD inline double log_recip(double d) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  const double r = __builtin_amdgcn_rcp(d);
-  return fma(fma(-d, r, 1.0), r, r);
-#else
-  return 1.0 / d;
-#endif
-}
-
-// log(x) for positive normal x; NaN in gives NaN out.
-FF_HD inline double log_pos(double x) {
+// log|x| for every x: -inf at 0, +inf at +-inf, NaN for NaN.  `tab` is
+// kLogTable (or a copy of it).
+FF_HD inline double log_abs(double x, const double *tab) {
   constexpr double kLn2Hi = 0x1.62e42fee00000p-1;   // ln 2, 32 bits
   constexpr double kLn2Lo = 0x1.a39ef35793c76p-33;  // ln 2 - kLn2Hi
-  constexpr double kSqrtHalf = 0.70710678118654752440;
   int k;
-  double m = frexp(x, &k);  // x = m 2^k, m in [0.5, 1)
-  if (m < kSqrtHalf) {
-    m += m;
-    --k;
-  }
-  const double f = m - 1.0;  // exact
-  const double s = f * log_recip(2.0 + f);
-  const double z = s * s;
-  double R = 0x1.2b6ada4e6cac0p-3;
-  R = fma(R, z, 0x1.39fcd91739d16p-3);
-  R = fma(R, z, 0x1.7462c0aff40b1p-3);
-  R = fma(R, z, 0x1.c71c62ba3869bp-3);
-  R = fma(R, z, 0x1.2492492e143e6p-2);
-  R = fma(R, z, 0x1.999999999519ap-2);
-  R = fma(R, z, 0x1.5555555555558p-1);
-  R *= z;
-  const double hfsq = 0.5 * f * f;
-  const double r = f - (hfsq - s * (hfsq + R));
+  // +-inf: m = inf gives r = inf and a result of +inf; NaN stays NaN
+  const double m = frexp(fabs(x), &k);  // |x| = m 2^k, m in [0.5, 1)
+  uint64_t bits;
+  __builtin_memcpy(&bits, &m, sizeof bits);
+  // byte offset of entry i = the top kLogTableBits fraction bits of m
+  const unsigned off = (unsigned)(bits >> (52 - kLogTableBits - 4)) &
+                       (((1u << kLogTableBits) - 1) << 4);
+  const double *e = (const double *)((const char *)tab + off);
+  const double invc = e[0], logc = e[1];
+  const double r = fma(m, invc, -1.0);
+  double q = fma(kLogQ3, r, kLogQ2);
+  q = fma(q, r, kLogQ1);
+  q = fma(q, r, kLogQ0);
+  const double y = fma(q, r * r, r);  // log1p(r)
   const double dk = (double)k;
-  return fma(dk, kLn2Hi, fma(dk, kLn2Lo, r));
+  const double l = fma(dk, kLn2Hi, logc) + fma(dk, kLn2Lo, y);
+  return x == 0.0 ? -INFINITY : l;
+}
+
+// log of a ratio clamped to [1e-4, 1e4] (ffKernel.f90:393-396, 399-402), taken as the
+// difference t = log|a| - log|b| clamped to [log(1e-4), log(1e4)] (the
+// double-rounded logarithms of the bounds as the reference computes them);
+// a NaN difference (0/0, inf/inf) passes through as the reference's NaN
+// ratio does.
+constexpr double kLogRatioLo = -0x1.26bb1bbb55515p+3;  // log(1e-4)
+constexpr double kLogRatioHi = 0x1.26bb1bbb55516p+3;   // log(1e4)
+FF_HD inline double clamp_log_ratio(double t) {
+  if (t < kLogRatioLo) t = kLogRatioLo;
+  if (t > kLogRatioHi) t = kLogRatioHi;
+  return t;
 }
 
 }  // namespace ff

@@ -310,7 +310,7 @@ int ff_oracle(int sem, const ffo_problem *p, ffo_c *achsDtemp,
         }
         continue;
       }
-      /* ffKernel.f90:335-384 */
+      /* ffKernel.f90:361-409 */
       const ffo_c cor_c = {(occ_band ? -zb : zb) - p->dFreqBrd[f].re,
                            -p->dFreqBrd[f].im};
       if (f != NF - 1)

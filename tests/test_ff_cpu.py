@@ -338,32 +338,70 @@ LOG_CHECK = r"""
 #include <cmath>
 #include <cstdio>
 #include <random>
-static double ulps(double got, long double want) {
-  const double w = (double)want;
-  const double u = std::nextafter(std::fabs(w), INFINITY) - std::fabs(w);
+using namespace gpp::ff;
+// |got - want| in units of ulp(max(|want|, ln 2)): the absolute error that a
+// difference of two logarithms inherits
+static double units(double got, long double want) {
+  const double w = std::fmax(std::fabs((double)want), 0.6931471805599453);
+  const double u = std::nextafter(w, INFINITY) - w;
   return (double)(std::fabs((long double)got - want) / u);
+}
+static long double clampl(long double r) {
+  return r < 1e-4L ? (long double)1e-4 : r > 1e4L ? (long double)1e4 : r;
 }
 int main() {
   std::mt19937_64 g(7);
-  std::uniform_real_distribution<double> wide(std::log(1e-4), std::log(1e4));
+  std::uniform_real_distribution<double> wide(std::log(1e-300), std::log(1e300));
   std::uniform_real_distribution<double> near(-1.0, 1.0);
-  double w = 0, n = 0;
+  std::uniform_real_distribution<double> u01(0.0, 1.0);
+  double w = 0, n = 0, sub = 0, d = 0;  // d: INFINITY once a difference fails
   for (int i = 0; i < 2000000; ++i) {
-    const double x = std::exp(wide(g));
-    w = std::fmax(w, ulps(gpp::ff::log_pos(x), logl((long double)x)));
-    const double y = 1.0 + near(g) * std::pow(10.0, -1 - i % 14);
-    if (y != 1.0) n = std::fmax(n, ulps(gpp::ff::log_pos(y), logl((long double)y)));
+    const double x = std::exp(wide(g)) * (i & 1 ? -1 : 1);
+    w = std::fmax(w, units(log_abs(x, kLogTable), logl(fabsl((long double)x))));
+    const double y = 1.0 + near(g) * std::pow(10.0, -1 - i % 15);
+    n = std::fmax(n, units(log_abs(y, kLogTable), logl((long double)y)));
+    const double s = std::ldexp(u01(g), -1022 - (i % 52));
+    if (s != 0) sub = std::fmax(sub, units(log_abs(s, kLogTable), logl((long double)s)));
+    // the interval term: clamp(log|a| - log|b|) against log(clamp|a/b|),
+    // a and b as |wx - zb| at adjacent grid points (ratios near 1 to far)
+    const double b = std::exp(wide(g) * 0.02);
+    const double a = b * (1.0 + near(g) * std::pow(10.0, 4 - i % 12));
+    const double t = clamp_log_ratio(log_abs(a, kLogTable) - log_abs(b, kLogTable));
+    const long double want = logl(clampl(fabsl((long double)a / b)));
+    const double big = std::fmax(std::fabs(std::log(std::fabs(a))), std::fabs(std::log(b)));
+    // two logarithms (1.3 units each), the subtraction's rounding and the
+    // bounds' own rounding: 3 ulp of the larger logarithm plus 1 of t
+    const double ub = std::fmax(big, 0.7), ut = std::fabs(t);
+    if (!(std::fabs((long double)t - want) <=
+          3.0 * (std::nextafter(ub, INFINITY) - ub) + (std::nextafter(ut, INFINITY) - ut)))
+      d = INFINITY;
   }
-  const double e1 = gpp::ff::log_pos(1.0), e2 = gpp::ff::log_pos(NAN);
-  std::printf("%.6f %.6f %g %d\n", w, n, e1, (int)std::isnan(e2));
+  const double lo = clamp_log_ratio(log_abs(0.0, kLogTable) - log_abs(2.0, kLogTable));
+  const double hi = clamp_log_ratio(log_abs(3.0, kLogTable) - log_abs(-0.0, kLogTable));
+  const double nan00 = clamp_log_ratio(log_abs(0.0, kLogTable) - log_abs(0.0, kLogTable));
+  std::printf("%.6f %.6f %.6f %d\n", w, n, sub, (int)(d < INFINITY));
+  std::printf("%d %d %d %d %d %d %d\n",
+              log_abs(0.0, kLogTable) == -INFINITY && log_abs(-0.0, kLogTable) == -INFINITY,
+              log_abs(INFINITY, kLogTable) == INFINITY && log_abs(-INFINITY, kLogTable) == INFINITY,
+              (int)std::isnan(log_abs(NAN, kLogTable)),
+              std::fabs(log_abs(1.0, kLogTable)) <= 0x1p-54 && log_abs(-1.0, kLogTable) == log_abs(1.0, kLogTable),
+              lo == std::log(1e-4) && hi == std::log(1e4),
+              (int)std::isnan(nan00),
+              kLogRatioLo == std::log(1e-4) && kLogRatioHi == std::log(1e4));
 }
 """
 
 
 def test_grid_kernel_logarithm_accuracy(tmp_path):
     """The logarithm of the frequency-grid kernel (csrc/ff_math.h, built for
-    the host with g++) against long double logl: at most 1.3 ulp over the
-    clamped range [1e-4, 1e4], at most 0.6 ulp within 0.1 of 1."""
+    the host with g++) against long double logl, as an absolute error in
+    units of ulp(max(|log x|, ln 2)): at most 1.3 over |x| in [1e-300,
+    1e300] and for subnormals, at most 0.6 within 0.1 of 1; the clamped
+    difference of two logarithms (the interval term) within 3 ulp of the
+    larger logarithm plus 1 ulp of the result of log(clamp(|a/b|)); special
+    values (log 0 = -inf, log inf = inf, NaN, |log 1| <= 2^-54, 0/x and x/0
+    clamped to the bounds, 0/0 NaN as the reference's ratio) and the clamp
+    bounds equal to log(1e-4) and log(1e4) as double."""
     import shutil
     if shutil.which("g++") is None:
         pytest.skip("no g++")
@@ -372,7 +410,30 @@ def test_grid_kernel_logarithm_accuracy(tmp_path):
     exe = tmp_path / "log_check"
     subprocess.run(["g++", "-O2", "-ffp-contract=off", "-I",
                     str(gppk.PKG_DIR / "csrc"), str(src), "-o", str(exe)], check=True)
-    w, n, one, nan = subprocess.run([str(exe)], capture_output=True, text=True,
-                                    check=True, timeout=120).stdout.split()
-    assert float(w) <= 1.3 and float(n) <= 0.6, (w, n)
-    assert float(one) == 0.0 and nan == "1"
+    line1, line2 = subprocess.run([str(exe)], capture_output=True, text=True,
+                                  check=True, timeout=120).stdout.splitlines()
+    w, n, sub, diff_ok = line1.split()
+    assert float(w) <= 1.3 and float(sub) <= 1.3 and float(n) <= 0.6, line1
+    assert diff_ok == "1"
+    assert line2.split() == ["1"] * 7, line2
+
+
+def test_log_table_is_generated():
+    """csrc/ff_log_table.h is what csrc/ff_log_table.py writes (table from
+    exact rational and 50-digit decimal arithmetic, each entry rounded
+    once), and its polynomial's economization bound is below 3.2e-13."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "ff_log_table", gppk.PKG_DIR / "csrc" / "ff_log_table.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    assert (gppk.PKG_DIR / "csrc" / "ff_log_table.h").read_text() == mod.header()
+    _, dropped = mod.economized_q()
+    assert float(dropped) < 3.2e-13
+    # every entry: invc the double nearest 1/c, logc the double nearest
+    # -ln(invc) (checked here with math.log on the exact invc, to 1 ulp)
+    import math
+    for i, (invc, logc) in enumerate(mod.table()):
+        c = 0.5 + (2 * i + 1) / 1024
+        assert invc == 1 / c
+        assert abs(logc + math.log(invc)) <= math.ulp(abs(logc))
