@@ -51,34 +51,6 @@ __device__ __forceinline__ double recip(double x) {
   return fma(q, fma(-x, q, 1.0), q);
 }
 
-// a / b formed as Complex.h:208 does: a*conj(b) / |b|^2.
-__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
-  const double r = recip(b.x * b.x + b.y * b.y);
-  return make_double2((a.x * b.x + a.y * b.y) * r, (a.y * b.x - a.x * b.y) * r);
-}
-
-// The same with b.y * b.y given (by2, rounded as above).
-__device__ __forceinline__ double2 cdiv_y2(double2 a, double2 b, double by2) {
-  const double r = recip(fma(b.x, b.x, by2));
-  return make_double2((a.x * b.x + a.y * b.y) * r, (a.y * b.x - a.x * b.y) * r);
-}
-
-// 1/d1 and 1/d2 from one reciprocal of d1*d2 (v_rcp_f64 + one Newton step);
-// each through recip() when the product is not a positive normal number.
-__device__ __forceinline__ void recip2(double d1, double d2, double &r1,
-                                       double &r2) {
-  const double p = d1 * d2;
-  if (__builtin_expect(!__builtin_amdgcn_class(p, 0x100), 0)) {
-    r1 = recip(d1);
-    r2 = recip(d2);
-    return;
-  }
-  double q = __builtin_amdgcn_rcp(p);
-  q = fma(q, fma(-p, q, 1.0), q);
-  r1 = q * d2;
-  r2 = q * d1;
-}
-
 // Coefficient table of a run of passes: for pass j (jobs[j]), row G' and
 // slot b < 8,
 //   cGj[(j*NG + G')*8 + b] = vcoul[igp] conj(aqsm[band_b][igp])
@@ -220,14 +192,20 @@ __global__ __launch_bounds__(kRedThreads) void slice_reduce_kernel(
 
 // The per-(n1, f) values of the grid terms that do not depend on iw, formed
 // once (one thread each) with the operations of the reference's loop body
-// (ffKernel.f90:363-382, ffKernel.cpp:200-206); row layout:
-//   0,1  i pref[f] schDt[n1][f]        2  zb - Re brd        3  cor shift
-//   4    Im brd      5  right = zb     6  left = zb[f-1]     7  (Im brd)^2
-//   8,9  i pref_zb avg (sch2 factor)   10,11 schDt[f-1]
-//   12,13 lin/(right-left)             14,15 lin = schDt[f] - schDt[f-1]
-// (ffKernel.cpp semantics: 3 = r1 r4 and 5 = r0 r4 instead.)
+// (ffKernel.f90:363-382, ffKernel.cpp:200-206), in two row arrays of
+// kRowHalf doubles per (n1, f) so that each grid loop reads one 64-byte row
+// per point with one scalar load.  Pole rows (both semantics):
+//   0,1  num = i pref[f] schDt[n1][f]     2  coh shift zb - Re brd
+//   3    (Im brd)^2       4  num.im Im brd    5  num.re Im brd
+//   6    cor shift (ffKernel.f90: -zb - Re brd for occupied bands, else 2)
+//   7    Im brd
+// (the unoccupied and ffKernel.cpp loops read 0-5 only).
+// Interval rows (ffKernel.f90 semantics; f = 0 holds only 0):
+//   0    right = zb[f]    1,2  i pref_zb avg (sch2 factor)
+//   3,4  pref_zb schDt[f-1]      5,6  pref_zb lin / (right - left)
 __global__ void freq_prep_kernel(FreqArgs a, int sem,
-                                 double *__restrict__ rows) {
+                                 double *__restrict__ prow,
+                                 double *__restrict__ irow) {
   const size_t n = (size_t)a.NB * a.nFreq;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -236,14 +214,24 @@ __global__ void freq_prep_kernel(FreqArgs a, int sem,
     const double zb = a.grid[f];
     const double2 br = a.brd[f];
     const double2 sdt = a.schdt[i];
-    double r[kRowDoubles];
-    r[0] = -a.pref[f] * sdt.y;
-    r[1] = a.pref[f] * sdt.x;
-    r[2] = zb - br.x;
-    r[3] = (occ ? -zb : zb) - br.x;
-    r[4] = br.y;
-    r[5] = zb;
-    r[7] = br.y * br.y;
+    double p[kRowHalf];
+    p[0] = -a.pref[f] * sdt.y;
+    p[1] = a.pref[f] * sdt.x;
+    p[2] = zb - br.x;
+    p[3] = br.y * br.y;
+    p[4] = p[1] * br.y;
+    p[5] = p[0] * br.y;
+    p[6] = (sem == 1 && occ) ? -zb - br.x : p[2];
+    p[7] = br.y;
+    double2 *o = (double2 *)(prow + i * kRowHalf);
+#pragma unroll
+    for (int k = 0; k < kRowHalf / 2; ++k)
+      o[k] = make_double2(p[2 * k], p[2 * k + 1]);
+    if (sem == 0) continue;
+    double q[kRowHalf];
+#pragma unroll
+    for (int k = 0; k < kRowHalf; ++k) q[k] = 0.0;
+    q[0] = zb;
     if (f > 0) {
       const double2 sp = a.schdt[i - 1];
       const double left = a.grid[f - 1];
@@ -251,31 +239,92 @@ __global__ void freq_prep_kernel(FreqArgs a, int sem,
           make_double2(0.5 * (sdt.x + sp.x), 0.5 * (sdt.y + sp.y));
       const double2 lin = make_double2(sdt.x - sp.x, sdt.y - sp.y);
       const double rl = 1.0 / (zb - left);
-      r[6] = left;
-      r[8] = -a.pref_zb * avg.y;
-      r[9] = a.pref_zb * avg.x;
-      r[10] = sp.x;
-      r[11] = sp.y;
-      r[12] = lin.x * rl;
-      r[13] = lin.y * rl;
-      r[14] = lin.x;
-      r[15] = lin.y;
-    } else {
-      r[6] = 0.0;
-#pragma unroll
-      for (int k = 8; k < kRowDoubles; ++k) r[k] = 0.0;
+      q[1] = -a.pref_zb * avg.y;
+      q[2] = a.pref_zb * avg.x;
+      q[3] = a.pref_zb * sp.x;
+      q[4] = a.pref_zb * sp.y;
+      q[5] = a.pref_zb * (lin.x * rl);
+      q[6] = a.pref_zb * (lin.y * rl);
     }
-    if (sem == 0) {
-      // ffKernel.cpp reads neither the cor shift nor the interval ends; its
-      // pole term uses the frequency-independent products i pref schDt *
-      // Im brd instead (rows 3 and 5 then hold r1 r4 and r0 r4).
-      r[3] = r[1] * r[4];
-      r[5] = r[0] * r[4];
-    }
-    double2 *o = (double2 *)(rows + i * kRowDoubles);
+    o = (double2 *)(irow + i * kRowHalf);
 #pragma unroll
-    for (int k = 0; k < kRowDoubles / 2; ++k)
-      o[k] = make_double2(r[2 * k], r[2 * k + 1]);
+    for (int k = 0; k < kRowHalf / 2; ++k)
+      o[k] = make_double2(q[2 * k], q[2 * k + 1]);
+  }
+}
+
+template <typename T>
+using ConstPtr = __attribute__((address_space(4))) const T *;
+
+// One 64-byte row (kRowHalf doubles) of a grid-term row array.
+struct Row {
+  double v[kRowHalf];
+};
+
+// The row at p, read whole at the point of the call.  p goes through an
+// empty asm so that the loop optimizer keeps the pointer at the row's
+// start (the loads then take non-negative immediate offsets, with no
+// address arithmetic per load); the constant address space (the rows are
+// written by an earlier launch and only read here) keeps them scalar.
+__device__ __forceinline__ Row load_row(const double *p) {
+  uint64_t a = (uint64_t)p;
+  asm volatile("" : "+s"(a));
+  const ConstPtr<double> q = (ConstPtr<double>)a;
+  Row r;
+#pragma unroll
+  for (int k = 0; k < kRowHalf; ++k) r.v[k] = q[k];
+  return r;
+}
+
+// -num / (x + i Im brd) = -num (x - i Im brd) q, q = 1/(x^2 + (Im brd)^2),
+// with the pole row's products num Im brd: re -= num.re (x q) + (num.im
+// Im brd) q, im -= num.im (x q) - (num.re Im brd) q.
+__device__ __forceinline__ void pole_term(const Row &r, double x, double q,
+                                          double &re, double &im) {
+  const double u = x * q;
+  re = fma(-r.v[0], u, re);
+  re = fma(-r.v[4], q, re);
+  im = fma(-r.v[1], u, im);
+  im = fma(r.v[5], q, im);
+}
+
+// The pair loops form -(x^2 + (Im brd)^2) = fma(x, -x, -(Im brd)^2): with
+// the negations the compiler keeps the three-operand fma, whose addend may
+// be the row's scalar register, instead of copying the scalar into the
+// destination of a two-operand fmac.  The reciprocals of the negated sums
+// are the negated reciprocals, and pole_term_neg takes them.
+__device__ __forceinline__ double neg_den(double x, double b2) {
+  return fma(x, -x, -b2);
+}
+
+// pole_term with nq = -q.
+__device__ __forceinline__ void pole_term_neg(const Row &r, double x,
+                                              double nq, double &re,
+                                              double &im) {
+  const double nu = x * nq;
+  re = fma(r.v[0], nu, re);
+  re = fma(r.v[4], nq, re);
+  im = fma(r.v[1], nu, im);
+  im = fma(-r.v[5], nq, im);
+}
+
+// 1/d1 and 1/d2 from one reciprocal of d1*d2 (v_rcp_f64 + one Newton
+// step); the test for a product that is not a positive normal number is
+// taken once per wave (a uniform branch), and the lanes that fail it redo
+// their pair with recip().
+__device__ __forceinline__ void recip2_wave(double d1, double d2, double &r1,
+                                            double &r2) {
+  const double p = d1 * d2;
+  double q = __builtin_amdgcn_rcp(p);
+  q = fma(q, fma(-p, q, 1.0), q);
+  r1 = q * d2;
+  r2 = q * d1;
+  const bool odd = !__builtin_amdgcn_class(p, 0x100);
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(odd) != 0, 0)) {
+    if (odd) {
+      r1 = recip(d1);
+      r2 = recip(d2);
+    }
   }
 }
 
@@ -294,7 +343,8 @@ __global__ void freq_prep_kernel(FreqArgs a, int sem,
 // and one logarithm per grid point, not two.
 template <int SEM, bool OCC>
 __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
-                                                   const double *__restrict__ rows,
+                                                   const double *__restrict__ prow,
+                                                   const double *__restrict__ irow,
                                                    double *__restrict__ part) {
   const int n1 = OCC ? (int)blockIdx.y : a.NV + (int)blockIdx.y;
   const int iw = blockIdx.x * blockDim.x + threadIdx.x;
@@ -304,117 +354,127 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
     for (int i = threadIdx.x; i < (2 << kLogTableBits); i += blockDim.x)
       logtab[i] = kLogTable[i];
     __syncthreads();
+    // whole waves past NE leave; lanes past NE in a live wave run along (the
+    // wave sums below need every lane) and store nothing
+    if (iw - (int)(threadIdx.x & (kWave - 1)) >= a.NE) return;
+  } else {
+    if (iw >= a.NE) return;
   }
-  if (iw >= a.NE) return;
   const int f0 = blockIdx.z * fchunk;
   const int f1 = min(a.nFreq, f0 + fchunk);
-  constexpr bool occ = OCC;
   const double ek = a.ekq[n1];
   const double wx = SEM == 0 ? a.fmin - ek + (iw - 1) * a.fstep
                              : a.fmin - ek + iw * a.fstep;
-  const double *rb = rows + (size_t)n1 * a.nFreq * kRowDoubles;
+  const double *pb = prow + (size_t)n1 * a.nFreq * kRowHalf;
   double achr = 0.0, achi = 0.0, cbr = 0.0, cbi = 0.0;
   double s2r = 0.0, s2i = 0.0, cr = 0.0, ci = 0.0;
-  // occupied lin3 argument: -freqevalmin - ekq + (iw-1)*freqevalstep (f90:403)
-  const double wocc = -a.fmin - ek + iw * a.fstep;
   int f = f0;
   if constexpr (SEM == 0) {
     // ffKernel.cpp: pole terms only; two grid points share a reciprocal
-    for (; f + 1 < f1; f += 2) {
-      const double *r = rb + (size_t)f * kRowDoubles;
-      const double *rn = r + kRowDoubles;
-      const double xa = wx - r[2], xb = wx - rn[2];
+    const double *p = pb + (size_t)f * kRowHalf;
+    for (; f + 1 < f1; f += 2, p += 2 * kRowHalf) {
+      const Row r = load_row(p), rn = load_row(p + kRowHalf);
+      const double xa = wx - r.v[2], xb = wx - rn.v[2];
       double qa, qb;
-      recip2(fma(xa, xa, r[7]), fma(xb, xb, rn[7]), qa, qb);
-      achr -= fma(r[0], xa, r[3]) * qa;
-      achi -= fma(r[1], xa, -r[5]) * qa;
-      achr -= fma(rn[0], xb, rn[3]) * qb;
-      achi -= fma(rn[1], xb, -rn[5]) * qb;
+      recip2_wave(neg_den(xa, r.v[3]), neg_den(xb, rn.v[3]), qa, qb);
+      pole_term_neg(r, xa, qa, achr, achi);
+      pole_term_neg(rn, xb, qb, achr, achi);
     }
-  }
-  if constexpr (SEM == 0) {
     for (; f < f1; ++f) {
-      const double *r = rb + (size_t)f * kRowDoubles;
-      const double2 t1 = cdiv_y2(make_double2(r[0], r[1]),
-                                 make_double2(wx - r[2], r[4]), r[7]);
-      achr -= t1.x;
-      achi -= t1.y;
+      const Row r = load_row(pb + (size_t)f * kRowHalf);
+      const double x = wx - r.v[2];
+      pole_term(r, x, recip(fma(x, x, r.v[3])), achr, achi);
     }
   } else {
     // Pole terms, f != nFreq-1 (f90:386-387); two grid points share a
-    // reciprocal.  Unoccupied bands: the cor pole is the coh pole (row 3 ==
+    // reciprocal.  Unoccupied bands: the cor pole is the coh pole (row 6 ==
     // row 2), so corb equals ach and is copied after the loops.
     const int fp1 = min(f1, a.nFreq - 1);
-    for (; f + 1 < fp1; f += 2) {
-      const double *r = rb + (size_t)f * kRowDoubles;
-      const double *rn = r + kRowDoubles;
-      const double xa = wx - r[2], xb = wx - rn[2];
+    const double *p = pb + (size_t)f * kRowHalf;
+    for (; f + 1 < fp1; f += 2, p += 2 * kRowHalf) {
+      const Row r = load_row(p), rn = load_row(p + kRowHalf);
+      const double xa = wx - r.v[2], xb = wx - rn.v[2];
       double qa, qb;
-      recip2(fma(xa, xa, r[7]), fma(xb, xb, rn[7]), qa, qb);
-      // -(i pref schDt) / (x + i brd.im) = -num conj(x + i brd.im) q
-      achr -= fma(r[0], xa, r[1] * r[4]) * qa;
-      achi -= fma(r[1], xa, -r[0] * r[4]) * qa;
-      achr -= fma(rn[0], xb, rn[1] * rn[4]) * qb;
-      achi -= fma(rn[1], xb, -rn[0] * rn[4]) * qb;
+      recip2_wave(neg_den(xa, r.v[3]), neg_den(xb, rn.v[3]), qa, qb);
+      pole_term_neg(r, xa, qa, achr, achi);
+      pole_term_neg(rn, xb, qb, achr, achi);
       if (OCC) {
-        const double ya = wx - r[3], yb = wx - rn[3];
-        recip2(fma(ya, ya, r[7]), fma(yb, yb, rn[7]), qa, qb);
-        cbr -= fma(r[0], ya, r[1] * r[4]) * qa;
-        cbi -= fma(r[1], ya, -r[0] * r[4]) * qa;
-        cbr -= fma(rn[0], yb, rn[1] * rn[4]) * qb;
-        cbi -= fma(rn[1], yb, -rn[0] * rn[4]) * qb;
+        const double ya = wx - r.v[6], yb = wx - rn.v[6];
+        recip2_wave(neg_den(ya, r.v[3]), neg_den(yb, rn.v[3]), qa, qb);
+        pole_term_neg(r, ya, qa, cbr, cbi);
+        pole_term_neg(rn, yb, qb, cbr, cbi);
       }
     }
     for (; f < fp1; ++f) {
-      const double *r = rb + (size_t)f * kRowDoubles;
-      const double2 num = make_double2(r[0], r[1]);
-      const double2 t1 = cdiv_y2(num, make_double2(wx - r[2], r[4]), r[7]);
-      achr -= t1.x;
-      achi -= t1.y;
+      const Row r = load_row(pb + (size_t)f * kRowHalf);
+      const double x = wx - r.v[2];
+      pole_term(r, x, recip(fma(x, x, r.v[3])), achr, achi);
       if (OCC) {
-        const double2 t2 = cdiv_y2(num, make_double2(wx - r[3], r[4]), r[7]);
-        cbr -= t2.x;
-        cbi -= t2.y;
+        const double y = wx - r.v[6];
+        pole_term(r, y, recip(fma(y, y, r.v[3])), cbr, cbi);
       }
     }
     // Interval terms, f != 0 (f90:391-409): log|(wx - right)/(wx - left)|
     // as the difference of log|wx - zb| at the interval's ends, carried from
-    // one grid point to the next (right = zb[f] = row 5, left = zb[f-1]).
+    // one grid point to the next (right = zb[f], left = zb[f-1]);
+    //   sch2 -= i pref_zb avg it
+    //   cor  -= i pref_zb ((schDt[f-1] + lin2 ar) it + lin),
+    // ar = wx - left (unoccupied: the previous point's wx - right) or
+    // -freqevalmin - ekq + (iw-1)*freqevalstep - left (occupied, f90:403),
+    // the lin part summed once below (it does not depend on iw), the rest
+    // with the row's pref_zb-scaled schDt[f-1] and lin2.
     const int fi0 = max(f0, 1);
     if (fi0 < f1) {
-      const double zl = rb[(size_t)(fi0 - 1) * kRowDoubles + 5];
-      double lw = log_abs(wx - zl, logtab);              // log|wx - left|
-      double lo = OCC ? log_abs(wx + zl, logtab) : 0.0;  // log|wx + left|
-      for (f = fi0; f < f1; ++f) {
-        const double *r = rb + (size_t)f * kRowDoubles;
-        const double right = r[5], left = r[6];
-        const double lr = log_abs(wx - right, logtab);
-        double it = -clamp_log_ratio(lr - lw);
-        lw = lr;
-        // sch2 -= i pref_zb * avg * it
-        s2r -= r[8] * it;
-        s2i -= r[9] * it;
-        double ar;
+      const double *ib = irow + (size_t)n1 * a.nFreq * kRowHalf;
+      const double wocc = -a.fmin - ek + iw * a.fstep;
+      const double zl = ib[(size_t)(fi0 - 1) * kRowHalf];
+      IntervalLog lw, lo;  // log|wx - zb|, log|wx + zb| (occupied)
+      double ar = OCC ? wocc - zl : wx - zl;
+      lw.start(wx - zl, logtab);
+      if (OCC) lo.start(wx + zl, logtab);
+      const double *p = ib + (size_t)fi0 * kRowHalf;
+      for (f = fi0; f < f1; ++f, p += kRowHalf) {
+        const Row rw = load_row(p);
+        const double *r = rw.v;
+        const double right = r[0];
+        const double xr = wx - right;
+        double it = -lw.step(xr, logtab);
+        s2r = fma(-r[1], it, s2r);
+        s2i = fma(-r[2], it, s2i);
+        double anext;
         if (OCC) {
-          const double lro = log_abs(wx + right, logtab);
-          it = clamp_log_ratio(lro - lo);
-          lo = lro;
-          ar = wocc - left;
+          it = lo.step(wx + right, logtab);
+          anext = wocc - right;
         } else {
-          ar = wx - left;
+          anext = xr;
         }
-        const double l3r = (r[10] + r[12] * ar) * it + r[14];
-        const double l3i = (r[11] + r[13] * ar) * it + r[15];
-        // cor -= i pref_zb * lin3
-        cr -= -a.pref_zb * l3i;
-        ci -= a.pref_zb * l3r;
+        const double u = ar * it;
+        cr = fma(r[4], it, cr);
+        cr = fma(r[6], u, cr);
+        ci = fma(-r[3], it, ci);
+        ci = fma(-r[5], u, ci);
+        ar = anext;
       }
+      // sum of lin = schDt[f] - schDt[f-1] over the block's intervals, the
+      // same for every iw: lanes take every 64th interval, then a wave sum
+      const double2 *sd = a.schdt + (size_t)n1 * a.nFreq;
+      double lre = 0.0, lim = 0.0;
+      for (int g = fi0 + (int)(threadIdx.x & (kWave - 1)); g < f1; g += kWave) {
+        const double2 sr = sd[g], sl = sd[g - 1];
+        lre += sr.x - sl.x;
+        lim += sr.y - sl.y;
+      }
+      lre = wave_sum(lre);
+      lim = wave_sum(lim);
+      cr = fma(a.pref_zb, lim, cr);
+      ci = fma(-a.pref_zb, lre, ci);
     }
   }
   if (SEM == 1 && !OCC) {
     cbr = achr;
     cbi = achi;
   }
+  if (iw >= a.NE) return;
   double *o =
       part + (((size_t)blockIdx.z * a.NB + n1) * a.NE + iw) * kFreqSlots;
   o[0] = achr;
@@ -506,23 +566,24 @@ hipError_t launch_freq(const FreqArgs &a, int sem, int splits, double *rows,
   const size_t n = (size_t)a.NB * a.nFreq;
   size_t blocks = (n + 255) / 256;
   if (blocks > 65536) blocks = 65536;
-  freq_prep_kernel<<<(unsigned)blocks, 256, 0, s>>>(a, sem, rows);
+  double *prow = rows, *irow = rows + n * kRowHalf;
+  freq_prep_kernel<<<(unsigned)blocks, 256, 0, s>>>(a, sem, prow, irow);
   const int fchunk = (a.nFreq + splits - 1) / splits;
   const int nocc = std::min(a.NV, a.NB), nun = a.NB - nocc;
   const unsigned gx = (unsigned)((a.NE + 255) / 256);
   if (nocc > 0) {
     const dim3 grid(gx, nocc, splits);
     if (sem == 0)
-      freq_kernel<0, true><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+      freq_kernel<0, true><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
     else
-      freq_kernel<1, true><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+      freq_kernel<1, true><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
   }
   if (nun > 0) {
     const dim3 grid(gx, nun, splits);
     if (sem == 0)
-      freq_kernel<0, false><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+      freq_kernel<0, false><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
     else
-      freq_kernel<1, false><<<grid, 256, 0, s>>>(a, fchunk, rows, part);
+      freq_kernel<1, false><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
   }
   return hipGetLastError();
 }

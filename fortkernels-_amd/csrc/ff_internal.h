@@ -17,7 +17,8 @@ constexpr int kRecSlots = 2 * kMaxJobBands;  // complex per band per record
 constexpr size_t kJobCgBytes = 16 * kMaxJobBands;  // per pass and G' row
 constexpr int kFreqSlots = 8;     // ach, corb, sch2 (grid part), cor (grid
                                   // part): complex each
-constexpr int kRowDoubles = 16;   // per (n1, f) grid-term constants
+constexpr int kRowDoubles = 16;   // per (n1, f) grid-term constants: a
+constexpr int kRowHalf = 8;       // pole row and an interval row of 8 each
 
 // One pass over one frequency slice of I_epsR or I_epsA for up to 8 bands:
 //   T[b] = sum_{G'} cG[band_b][G'] sum_G aqsn[band_b][G] * slice[G'][G]

@@ -30,12 +30,20 @@
 #define FF_HD
 #endif
 
+// True when `c` holds on any lane of the wave (on the host: c).  A branch on
+// it is uniform: one scalar compare, no exec-mask bookkeeping.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FF_ANY_LANE(c) (__builtin_amdgcn_ballot_w64(c) != 0)
+#else
+#define FF_ANY_LANE(c) (c)
+#endif
+
 namespace gpp {
 namespace ff {
 
-// log|x| for every x: -inf at 0, +inf at +-inf, NaN for NaN.  `tab` is
-// kLogTable (or a copy of it).
-FF_HD inline double log_abs(double x, const double *tab) {
+// log|x| for x != 0: +inf at +-inf, NaN for NaN, a finite value of no
+// meaning at 0.  `tab` is kLogTable (or a copy of it).
+FF_HD inline double log_abs_nonzero(double x, const double *tab) {
   constexpr double kLn2Hi = 0x1.62e42fee00000p-1;   // ln 2, 32 bits
   constexpr double kLn2Lo = 0x1.a39ef35793c76p-33;  // ln 2 - kLn2Hi
   int k;
@@ -54,15 +62,20 @@ FF_HD inline double log_abs(double x, const double *tab) {
   q = fma(q, r, kLogQ0);
   const double y = fma(q, r * r, r);  // log1p(r)
   const double dk = (double)k;
-  const double l = fma(dk, kLn2Hi, logc) + fma(dk, kLn2Lo, y);
+  return fma(dk, kLn2Hi, logc) + fma(dk, kLn2Lo, y);
+}
+
+// log|x| for every x: -inf at 0, +inf at +-inf, NaN for NaN.
+FF_HD inline double log_abs(double x, const double *tab) {
+  const double l = log_abs_nonzero(x, tab);
   return x == 0.0 ? -INFINITY : l;
 }
 
-// log of a ratio clamped to [1e-4, 1e4] (ffKernel.f90:393-396, 399-402), taken as the
-// difference t = log|a| - log|b| clamped to [log(1e-4), log(1e4)] (the
-// double-rounded logarithms of the bounds as the reference computes them);
-// a NaN difference (0/0, inf/inf) passes through as the reference's NaN
-// ratio does.
+// log of a ratio clamped to [1e-4, 1e4] (ffKernel.f90:393-396, 399-402),
+// taken as the difference t = log|a| - log|b| clamped to [log(1e-4),
+// log(1e4)] (the double-rounded logarithms of the bounds as the reference
+// computes them); a NaN difference (0/0, inf/inf) passes through as the
+// reference's NaN ratio does.
 constexpr double kLogRatioLo = -0x1.26bb1bbb55515p+3;  // log(1e-4)
 constexpr double kLogRatioHi = 0x1.26bb1bbb55516p+3;   // log(1e4)
 FF_HD inline double clamp_log_ratio(double t) {
@@ -70,6 +83,36 @@ FF_HD inline double clamp_log_ratio(double t) {
   if (t > kLogRatioHi) t = kLogRatioHi;
   return t;
 }
+
+// The interval logarithms of consecutive grid points, log|a_f / a_{f-1}|
+// clamped as above, a_f = wx - zb[f] (or wx + zb[f]): one logarithm per
+// point, carried to the next interval.  Most differences lie inside the
+// bounds and neither end is zero, and for them the step is one compare
+// (two with the zero test); the rest take the exact form: log 0 = -inf
+// (so x/0 and 0/x clamp to the bounds and 0/0 is NaN), the clamp, NaN
+// passed through.
+struct IntervalLog {
+  double l;  // log|a_{f-1}| (log_abs_nonzero)
+  bool z;    // a_{f-1} == 0
+  FF_HD void start(double a, const double *tab) {
+    l = log_abs_nonzero(a, tab);
+    z = a == 0.0;
+  }
+  FF_HD double step(double a, const double *tab) {
+    const double lr = log_abs_nonzero(a, tab);
+    const bool zr = a == 0.0;
+    double t = lr - l;
+    // -kLogRatioLo < kLogRatioHi: |t| <= -kLogRatioLo is inside both bounds
+    const bool exact = !(fabs(t) <= -kLogRatioLo) | zr | z;
+    if (FF_ANY_LANE(exact)) {
+      const double te = clamp_log_ratio((zr ? -INFINITY : lr) - (z ? -INFINITY : l));
+      t = exact ? te : t;
+    }
+    l = lr;
+    z = zr;
+    return t;
+  }
+};
 
 }  // namespace ff
 }  // namespace gpp

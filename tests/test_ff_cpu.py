@@ -376,11 +376,41 @@ int main() {
           3.0 * (std::nextafter(ub, INFINITY) - ub) + (std::nextafter(ut, INFINITY) - ut)))
       d = INFINITY;
   }
+  // the kernel's recurrence (IntervalLog) against the f90's own steps,
+  // intfact = clamp(|a/b|) and log(intfact) in double, over grids with
+  // wx on grid points, repeated points, huge and non-finite ones
+  int seq_ok = 1;
+  const double grid[] = {0.0, 0.5, 1.0, 1.0, 1.5, 2.0, 2.0 + 1e-9, 3.0, 3.0 + 1e-14,
+                         7.25, 1e300, INFINITY, 4.0, -2.0, NAN, 5.0, 6.0};
+  const int ng = sizeof grid / sizeof grid[0];
+  const double wxs[] = {1.0, 2.0, 0.25, -3.0, 3.0, 7.25, 2.0 + 5e-10};
+  for (double wx : wxs)
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      IntervalLog st;
+      st.start(wx + sgn * grid[0], kLogTable);
+      for (int f = 1; f < ng; ++f) {
+        const double a = wx + sgn * grid[f], b = wx + sgn * grid[f - 1];
+        const double t = st.step(a, kLogTable);
+        double ratio = std::fabs(a / b);
+        if (ratio < 1e-4) ratio = 1e-4;
+        if (ratio > 1e4) ratio = 1e4;
+        const double ref = std::log(ratio);
+        if (std::isnan(ref)) {
+          seq_ok &= std::isnan(t);
+          continue;
+        }
+        const double big = std::fmax(std::fmax(std::fabs(std::log(std::fabs(a))),
+                                               std::fabs(std::log(std::fabs(b)))), 0.7);
+        const double tol = std::isfinite(big)
+            ? 3.0 * (std::nextafter(big, INFINITY) - big) + 0x1p-50 : 0.0;
+        seq_ok &= std::fabs(t - ref) <= tol;
+      }
+    }
   const double lo = clamp_log_ratio(log_abs(0.0, kLogTable) - log_abs(2.0, kLogTable));
   const double hi = clamp_log_ratio(log_abs(3.0, kLogTable) - log_abs(-0.0, kLogTable));
   const double nan00 = clamp_log_ratio(log_abs(0.0, kLogTable) - log_abs(0.0, kLogTable));
   std::printf("%.6f %.6f %.6f %d\n", w, n, sub, (int)(d < INFINITY));
-  std::printf("%d %d %d %d %d %d %d\n",
+  std::printf("%d %d %d %d %d %d %d %d\n", seq_ok,
               log_abs(0.0, kLogTable) == -INFINITY && log_abs(-0.0, kLogTable) == -INFINITY,
               log_abs(INFINITY, kLogTable) == INFINITY && log_abs(-INFINITY, kLogTable) == INFINITY,
               (int)std::isnan(log_abs(NAN, kLogTable)),
@@ -401,7 +431,11 @@ def test_grid_kernel_logarithm_accuracy(tmp_path):
     larger logarithm plus 1 ulp of the result of log(clamp(|a/b|)); special
     values (log 0 = -inf, log inf = inf, NaN, |log 1| <= 2^-54, 0/x and x/0
     clamped to the bounds, 0/0 NaN as the reference's ratio) and the clamp
-    bounds equal to log(1e-4) and log(1e4) as double."""
+    bounds equal to log(1e-4) and log(1e4) as double; and the kernel's
+    recurrence over consecutive grid points (IntervalLog: one logarithm per
+    point, exact form only for differences outside the bounds or a zero
+    end) against the f90's clamp(|a/b|) and log over grids holding zeros,
+    repeated points and non-finite values."""
     import shutil
     if shutil.which("g++") is None:
         pytest.skip("no g++")
@@ -415,7 +449,7 @@ def test_grid_kernel_logarithm_accuracy(tmp_path):
     w, n, sub, diff_ok = line1.split()
     assert float(w) <= 1.3 and float(sub) <= 1.3 and float(n) <= 0.6, line1
     assert diff_ok == "1"
-    assert line2.split() == ["1"] * 7, line2
+    assert line2.split() == ["1"] * 8, line2
 
 
 def test_log_table_is_generated():
