@@ -22,6 +22,10 @@ The JSON line also carries:
                 average HIP-event duration, against the FP64 VALU peak;
                 traffic = HBM bytes per launch measured by rocprofv3 (from
                 profiles/traffic_*.json when present, else null);
+                valu_issue = the issue-level VALU utilisation beside it
+                (instructions per term from rocprofv3 counters, committed in
+                profiles/valu_issue.json, at this run's term rate; see
+                valu_issue());
   cpu_baseline  the reference mini-app (oracle/_ref/gppKer_xl, built from
                 gppKerOpenMPFor_xl.cpp: gppKer.cpp without ITT calls and with
                 its non-occupied mask active, which keeps every term of its
@@ -139,6 +143,28 @@ def _load(name, path, pkg=False):
     sys.modules[name] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+def valu_issue(key, terms_per_s):
+    """Issue-level VALU utilisation of a bench line's dominant kernel: its
+    VALU instructions per term (rocprofv3 SQ_INSTS_VALU, committed in
+    profiles/valu_issue.json with the profile each comes from) times the
+    SIMD cycles one takes at full rate, at the kernel's term rate measured
+    in this run, over the chip's SIMD cycles at the peak clock (1024 SIMDs x
+    2.4 GHz).  None when the kernel has no committed counts."""
+    f = ROOT / "profiles" / "valu_issue.json"
+    if not f.exists():
+        return None
+    t = json.loads(f.read_text()).get(key)
+    if not t:
+        return None
+    cycles = t["valu_insts_per_term"] / 64.0 * t["simd_cycles_per_inst"] * terms_per_s
+    return {"kernel": key, "valu_insts_per_term": t["valu_insts_per_term"],
+            "simd_cycles_per_inst": t["simd_cycles_per_inst"],
+            "frac_at_peak_clock": cycles / (1024 * 2.4e9),
+            "busy_measured": t.get("busy_measured"),
+            "clock_ghz_measured": t.get("clock_ghz_measured"),
+            "source": t["source"]}
 
 
 def log(*a):
@@ -435,7 +461,9 @@ def run_single(args, name):
                      "frac": achieved / peak, "traffic": traffic,
                      "kernel": f"unocc_{best}",
                      "flops_per_launch": flops_launch,
-                     "avg_launch_ms": b["unocc_kernel_ms"]},
+                     "avg_launch_ms": b["unocc_kernel_ms"],
+                     "valu_issue": None if best == FACT else valu_issue(
+                         f"unocc_{best}", unocc_terms / (b["unocc_kernel_ms"] * 1e-3))},
         "paths": {p: {"terms_per_s": r["terms_per_s"], "ms_per_step": r["ms_per_step"],
                       "note": FACT_NOTE if p == FACT else None,
                       "unocc_kernel_ms": r["unocc_kernel_ms"],
@@ -636,7 +664,9 @@ def run_ff(args, name):
                 "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
                 "kernel": "ff frequency-grid terms (constants, grid, reduction)",
                 "flops_per_launch": fl, "avg_launch_ms": r["freq_kernel_ms"],
-                "note": "division and logarithm counted as 1 flop each"}
+                "note": "division and logarithm counted as 1 flop each",
+                "valu_issue": valu_issue(f"ff_grid_{sem}", nb * nf * ne /
+                                         (r["freq_kernel_ms"] * 1e-3))}
 
     line = {
         "metric": FF_METRIC, "value": runs["cpp"]["evals_per_s"], "unit": "evals/s",
@@ -735,6 +765,20 @@ def run_batch(args, name="batch"):
         "one_by_one_fp64": {"terms_per_s": terms / t1, "ms": t1 * 1e3},
         "batch_speedup_vs_one_by_one_fp64": out[best]["terms_per_s"] / (terms / t1),
     }
+    # Roofline of the batch's non-occupied kernels: their launches sit in
+    # one graph with the rest of the evaluation, so the whole step's time
+    # stands in for theirs (achieved and valu_issue are lower bounds).
+    unocc = sum((d["number_bands"] - min(d["nvband"], d["number_bands"])) *
+                d["ngpown"] * d["ncouls"] * 3 for d in insts)
+    step_s = out[best]["ms_per_step"] * 1e-3
+    peak = PEAK_FP32_TFLOPS if best == "mixed" else PEAK_FP64_TFLOPS
+    ach = FLOP_UNOCC * unocc / step_s / 1e12
+    line["roofline"] = {"bound": "valu", "achieved": ach, "peak": peak,
+                        "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                        "kernel": f"unocc_{best} (whole batch step)",
+                        "flops_per_launch": FLOP_UNOCC * unocc,
+                        "avg_launch_ms": out[best]["ms_per_step"],
+                        "valu_issue": valu_issue(f"unocc_{best}", unocc / step_s)}
     h.close()
     return line
 
