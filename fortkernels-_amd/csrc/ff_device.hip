@@ -337,23 +337,18 @@ __device__ __forceinline__ void recip2_wave(double d1, double d2, double &r1,
 // A band's rows are wave-uniform (scalar loads).  blockIdx.z picks a
 // contiguous range of f (the grid is split so that small NE*NB still fills
 // the chip); part[((z*NB + n1)*NE + iw)*8].
-// OCC: the launch covers occupied bands only (n1 = blockIdx.y) or
-// unoccupied ones only (n1 = NV + blockIdx.y), so the band kind is a
-// compile-time constant and the unoccupied loop carries one pole division
-// and one logarithm per grid point, not two.
+// OCC: the body for occupied bands (n1 < NV) or unoccupied ones, a
+// compile-time constant, so that the unoccupied loop carries one pole
+// division and one logarithm per grid point, not two.
 template <int SEM, bool OCC>
-__global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
-                                                   const double *__restrict__ prow,
-                                                   const double *__restrict__ irow,
-                                                   double *__restrict__ part) {
-  const int n1 = OCC ? (int)blockIdx.y : a.NV + (int)blockIdx.y;
+__device__ __forceinline__ void freq_body(const FreqArgs &a, int fchunk,
+                                          const double *__restrict__ prow,
+                                          const double *__restrict__ irow,
+                                          double *__restrict__ part,
+                                          const double *logtab) {
+  const int n1 = (int)blockIdx.y;
   const int iw = blockIdx.x * blockDim.x + threadIdx.x;
-  // the logarithm's table (4 KiB), read per lane from LDS
-  __shared__ double logtab[SEM == 1 ? 2 << kLogTableBits : 1];
   if constexpr (SEM == 1) {
-    for (int i = threadIdx.x; i < (2 << kLogTableBits); i += blockDim.x)
-      logtab[i] = kLogTable[i];
-    __syncthreads();
     // whole waves past NE leave; lanes past NE in a live wave run along (the
     // wave sums below need every lane) and store nothing
     if (iw - (int)(threadIdx.x & (kWave - 1)) >= a.NE) return;
@@ -487,6 +482,30 @@ __global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
   o[7] = ci;
 }
 
+// One launch over all bands (n1 = blockIdx.y): the occupied bands' blocks
+// take the OCC body, the rest the unoccupied one (a uniform branch per
+// block), so the two kinds share the chip instead of running one launch
+// after the other.  ffKernel.cpp semantics has one body for both.
+template <int SEM>
+__global__ __launch_bounds__(256) void freq_kernel(FreqArgs a, int fchunk,
+                                                   const double *__restrict__ prow,
+                                                   const double *__restrict__ irow,
+                                                   double *__restrict__ part) {
+  // the logarithm's table (4 KiB), read per lane from LDS
+  __shared__ double logtab[SEM == 1 ? 2 << kLogTableBits : 1];
+  if constexpr (SEM == 1) {
+    for (int i = threadIdx.x; i < (2 << kLogTableBits); i += blockDim.x)
+      logtab[i] = kLogTable[i];
+    __syncthreads();
+    if ((int)blockIdx.y < a.NV)
+      freq_body<1, true>(a, fchunk, prow, irow, part, logtab);
+    else
+      freq_body<1, false>(a, fchunk, prow, irow, part, logtab);
+  } else {
+    freq_body<0, false>(a, fchunk, prow, irow, part, logtab);
+  }
+}
+
 // One wave per iw: lane = (row stream lane>>3, slot lane&7); each lane sums
 // rows sub, sub+8, ... of part in order, then the eight streams are added
 // in a fixed order (xor 8, 16, 32).
@@ -569,22 +588,11 @@ hipError_t launch_freq(const FreqArgs &a, int sem, int splits, double *rows,
   double *prow = rows, *irow = rows + n * kRowHalf;
   freq_prep_kernel<<<(unsigned)blocks, 256, 0, s>>>(a, sem, prow, irow);
   const int fchunk = (a.nFreq + splits - 1) / splits;
-  const int nocc = std::min(a.NV, a.NB), nun = a.NB - nocc;
-  const unsigned gx = (unsigned)((a.NE + 255) / 256);
-  if (nocc > 0) {
-    const dim3 grid(gx, nocc, splits);
-    if (sem == 0)
-      freq_kernel<0, true><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
-    else
-      freq_kernel<1, true><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
-  }
-  if (nun > 0) {
-    const dim3 grid(gx, nun, splits);
-    if (sem == 0)
-      freq_kernel<0, false><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
-    else
-      freq_kernel<1, false><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
-  }
+  const dim3 grid((unsigned)((a.NE + 255) / 256), a.NB, splits);
+  if (sem == 0)
+    freq_kernel<0><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
+  else
+    freq_kernel<1><<<grid, 256, 0, s>>>(a, fchunk, prow, irow, part);
   return hipGetLastError();
 }
 
