@@ -27,6 +27,7 @@
 #include "../../include/ff_sigma.h"
 #include "ff_internal.h"
 #include "gpp_handle.h"
+#include "host_pool.h"
 
 using gpp::Arena;
 using gpp::ensure;
@@ -143,6 +144,47 @@ struct Cplx {
     im += i;
   }
 };
+
+}  // namespace
+
+// Host planning buffers of one handle, kept between calls so that a
+// repeated evaluation neither allocates nor touches fresh pages.
+struct gpp::ff::PlanCache {
+  gpp::HostPool pool;
+  std::vector<int> need;  // [(kind*nF + f)*NB + band] -> job*8 + slot, or -1
+  std::vector<Fact> fasx, fcor;
+  std::vector<signed char> kasx, pcor;
+  std::vector<int> row_jobs, job_off, slice_off;  // per (kind, f) row
+  std::vector<SliceJob> jobs;
+  std::vector<int> job_slice;
+  std::vector<std::pair<int, int>> slices;  // (kind, f)
+};
+
+void gpp::ff::free_plan_cache(PlanCache *c) { delete c; }
+
+namespace {
+
+// Sizes the handle's planning buffers (need filled with -1); GPP_ENOMEM
+// when the host allocation fails.
+int plan_buffers(gpp_handle h, size_t nneed, size_t nasx, size_t ncor,
+                 size_t nrows, gpp::ff::PlanCache **out) {
+  try {
+    if (!h->ff_plan) h->ff_plan = new gpp::ff::PlanCache;
+    gpp::ff::PlanCache &c = *h->ff_plan;
+    c.need.assign(nneed, -1);
+    c.fasx.resize(nasx);
+    c.kasx.resize(nasx);
+    c.fcor.resize(ncor);
+    c.pcor.resize(ncor);
+    c.row_jobs.resize(nrows);
+    c.job_off.resize(nrows + 1);
+    c.slice_off.resize(nrows + 1);
+  } catch (...) {
+    return set_err(h, GPP_ENOMEM, "host planning buffers (%zu entries)", nneed);
+  }
+  *out = h->ff_plan;
+  return GPP_OK;
+}
 
 int validate(gpp_handle h, const ff_problem *p, const ff_options *o,
              int parts) {
@@ -325,28 +367,69 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   if (do_ach && (rc = launch_grid_terms(h, p, host, sem))) return rc;
 
   // ---- plan: which (kind, slice, band) sums are needed
-  // need[(kind*nF + f)*NB + band] -> job*8 + slot, or -1
-  std::vector<int> need((size_t)2 * nF * NB, -1);
+  gpp::ff::PlanCache *pc = nullptr;
+  if ((rc = plan_buffers(h, (size_t)2 * nF * NB, (size_t)NV * NE,
+                         (size_t)NB * NE, (size_t)2 * nF, &pc)))
+    return rc;
+  std::vector<int> &need = pc->need;
+  // Written only when it changes: the bands of one (kind, f) share a cache
+  // line, and the planning threads mark the same (kind, f) over and over
+  // (every iw, in the ffKernel.cpp semantics); two tasks of one band may
+  // mark one entry, hence the relaxed atomic accesses.
   auto mark = [&](int kind, int f, int band) {
-    need[((size_t)kind * nF + f) * NB + band] = 0;
+    int *e = &need[((size_t)kind * nF + f) * NB + band];
+    if (__atomic_load_n(e, __ATOMIC_RELAXED) != 0)
+      __atomic_store_n(e, 0, __ATOMIC_RELAXED);
   };
   const bool sums_possible = NB > 0 && NG > 0;
   const int achs_slice = sem == FF_SEM_CPP ? 1 : 0;
   if ((parts & FF_PART_ACHS) && sums_possible)
     for (int b = 0; b < NB; ++b) mark(kR, achs_slice, b);
-  // asx: [NV][NE] weights (C++: the same for every iw)
-  std::vector<Fact> fasx((size_t)NV * NE);
-  std::vector<signed char> kasx((size_t)NV * NE, kR);
-  if ((parts & FF_PART_ASX) && sums_possible)
-    for (int b = 0; b < NV; ++b) {
-      Walker walk;
-      for (int iw = 0; iw < NE; ++iw) {
-        const double wx = sem == FF_SEM_CPP
-                              ? p->freqevalmin - ekq[b] + p->freqevalstep
-                              : p->freqevalmin - ekq[b] + iw * p->freqevalstep;
-        Fact fc = (sem == FF_SEM_CPP && iw > 0)
-                      ? fasx[(size_t)b * NE]
-                      : fact(sem, 1, true, wx, nF, grid.data(), sorted, &walk);
+  // asx: [NV][NE] weights (C++: the same for every iw); cor: [NB][NE].
+  std::vector<Fact> &fasx = pc->fasx, &fcor = pc->fcor;
+  std::vector<signed char> &kasx = pc->kasx, &pcor = pc->pcor;
+  const bool plan_asx = (parts & FF_PART_ASX) && sums_possible;
+  const bool plan_cor = (parts & FF_PART_COR) && sums_possible;
+  // Tasks of kPlanChunk evaluation frequencies of one band and one loop,
+  // so that the occupied bands' second loop does not set the time; each
+  // task starts its grid walk with a binary search.
+  constexpr int kPlanChunk = 2048;
+  const int nch = (NE + kPlanChunk - 1) / kPlanChunk;
+  const int nt_asx = plan_asx ? NV * nch : 0, nt_cor = plan_cor ? NB * nch : 0;
+  auto wx_of = [&](int b, int iw) {
+    return sem == FF_SEM_CPP ? p->freqevalmin - ekq[b] + p->freqevalstep
+                             : p->freqevalmin - ekq[b] + iw * p->freqevalstep;
+  };
+  // ffKernel.cpp semantics: wx does not depend on iw, so each band's two
+  // facts are found once (compute_fact scans the whole grid).
+  std::vector<Fact> cpp_fact;
+  if (sem == FF_SEM_CPP) {
+    try {
+      cpp_fact.resize((size_t)2 * NB);
+    } catch (...) {
+      return set_err(h, GPP_ENOMEM, "host planning buffers");
+    }
+    for (int b = 0; b < NB; ++b) {
+      if (plan_asx && b < NV)
+        cpp_fact[2 * b] = fact(sem, 1, true, wx_of(b, 0), nF, grid.data(), sorted, nullptr);
+      if (plan_cor)
+        cpp_fact[2 * b + 1] = fact(sem, 2, b < NV, wx_of(b, 0), nF, grid.data(), sorted, nullptr);
+    }
+  }
+  auto plan_task = [&](int t) {
+    const bool asx_task = t < nt_asx;
+    const int u = asx_task ? t : t - nt_asx;
+    const int b = u / nch, i0 = (u % nch) * kPlanChunk;
+    const int i1 = std::min(NE, i0 + kPlanChunk);
+    Walker walk;
+    if (sorted)
+      walk.k = search_half_open(std::fabs(wx_of(b, i0)), nF, grid.data(), true);
+    if (asx_task) {
+      for (int iw = i0; iw < i1; ++iw) {
+        const double wx = wx_of(b, iw);
+        const Fact fc = sem == FF_SEM_CPP
+                            ? cpp_fact[2 * b]
+                            : fact(sem, 1, true, wx, nF, grid.data(), sorted, &walk);
         const int kind = (sem == FF_SEM_F90 && wx < 0.0) ? kA : kR;
         fasx[(size_t)b * NE + iw] = fc;
         kasx[(size_t)b * NE + iw] = (signed char)kind;
@@ -355,21 +438,13 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
           mark(kind, fc.f + 1, b);
         }
       }
-    }
-  // cor: [NB][NE]
-  std::vector<Fact> fcor((size_t)NB * NE);
-  std::vector<signed char> pcor((size_t)NB * NE, 0);
-  if ((parts & FF_PART_COR) && sums_possible)
-    for (int b = 0; b < NB; ++b) {
-      Walker walk;
-      for (int iw = 0; iw < NE; ++iw) {
-        const double wx = sem == FF_SEM_CPP
-                              ? p->freqevalmin - ekq[b] + p->freqevalstep
-                              : p->freqevalmin - ekq[b] + iw * p->freqevalstep;
-        const bool occ = b < NV;
-        Fact fc = (sem == FF_SEM_CPP && iw > 0)
-                      ? fcor[(size_t)b * NE]
-                      : fact(sem, 2, occ, wx, nF, grid.data(), sorted, &walk);
+    } else {
+      const bool occ = b < NV;
+      for (int iw = i0; iw < i1; ++iw) {
+        const double wx = wx_of(b, iw);
+        const Fact fc = sem == FF_SEM_CPP
+                            ? cpp_fact[2 * b + 1]
+                            : fact(sem, 2, occ, wx, nF, grid.data(), sorted, &walk);
         fcor[(size_t)b * NE + iw] = fc;
         pcor[(size_t)b * NE + iw] =
             (signed char)(sem == FF_SEM_CPP ? wx > 0.0 : wx >= 0.0);
@@ -380,33 +455,55 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
           }
       }
     }
+  };
+  parallel_ranges(&pc->pool, nt_asx + nt_cor, (long long)(NV + NB) * NE, 20000,
+                  plan_task);
 
-  // ---- pack passes: up to 8 bands per (kind, slice)
-  std::vector<SliceJob> jobs;
-  std::vector<int> job_slice;  // index into the staged slices (host memory)
-  std::vector<std::pair<int, int>> slices;  // (kind, f)
-  for (int kind = kR; kind <= kA; ++kind)
-    for (int f = 0; f < nF; ++f) {
-      int *row = &need[((size_t)kind * nF + f) * NB];
-      bool any = false;
-      for (int b = 0; b < NB && !any; ++b) any = row[b] >= 0;
-      if (!any) continue;
-      const int si = (int)slices.size();
-      slices.push_back({kind, f});
-      SliceJob j{};
+  // ---- pack passes: up to 8 bands per (kind, slice), numbered in (kind,
+  // f) order: count each row's passes, take the offsets, then fill the
+  // rows on several host threads (each row writes its own range).
+  constexpr int kRowBlock = 256;
+  const int nrows = 2 * nF, nrb = (nrows + kRowBlock - 1) / kRowBlock;
+  parallel_ranges(&pc->pool, nrb, (long long)nrows * NB, 20000, [&](int blk) {
+    for (int r = blk * kRowBlock; r < std::min(nrows, (blk + 1) * kRowBlock); ++r) {
+      const int *row = &need[(size_t)r * NB];
+      int nb = 0;
+      for (int b = 0; b < NB; ++b) nb += row[b] >= 0;
+      pc->row_jobs[r] = (nb + gpp::ff::kMaxJobBands - 1) / gpp::ff::kMaxJobBands;
+    }
+  });
+  pc->job_off[0] = pc->slice_off[0] = 0;
+  for (int r = 0; r < nrows; ++r) {
+    pc->job_off[r + 1] = pc->job_off[r] + pc->row_jobs[r];
+    pc->slice_off[r + 1] = pc->slice_off[r] + (pc->row_jobs[r] > 0);
+  }
+  try {
+    pc->jobs.assign(pc->job_off[nrows], SliceJob{});
+    pc->job_slice.resize(pc->job_off[nrows]);
+    pc->slices.resize(pc->slice_off[nrows]);
+  } catch (...) {
+    return set_err(h, GPP_ENOMEM, "host planning buffers (%d passes)",
+                   pc->job_off[nrows]);
+  }
+  std::vector<SliceJob> &jobs = pc->jobs;
+  std::vector<int> &job_slice = pc->job_slice;  // index into the staged slices
+  std::vector<std::pair<int, int>> &slices = pc->slices;  // (kind, f)
+  parallel_ranges(&pc->pool, nrb, (long long)nrows * NB, 20000, [&](int blk) {
+    for (int r = blk * kRowBlock; r < std::min(nrows, (blk + 1) * kRowBlock); ++r) {
+      if (pc->row_jobs[r] == 0) continue;
+      const int si = pc->slice_off[r];
+      slices[si] = {r / nF, r % nF};
+      int *row = &need[(size_t)r * NB];
+      int j = pc->job_off[r];
       for (int b = 0; b < NB; ++b) {
         if (row[b] < 0) continue;
-        if (j.nb == gpp::ff::kMaxJobBands) {
-          jobs.push_back(j);
-          job_slice.push_back(si);
-          j = SliceJob{};
-        }
-        row[b] = (int)jobs.size() * gpp::ff::kMaxJobBands + j.nb;
-        j.band[j.nb++] = b;
+        if (jobs[j].nb == gpp::ff::kMaxJobBands) ++j;
+        row[b] = j * gpp::ff::kMaxJobBands + jobs[j].nb;
+        jobs[j].band[jobs[j].nb++] = b;
+        job_slice[j] = si;
       }
-      jobs.push_back(j);
-      job_slice.push_back(si);
     }
+  });
   const int njobs = (int)jobs.size();
   const double plan_ms =
       std::chrono::duration<double, std::milli>(clk::now() - t_plan).count();
@@ -550,30 +647,38 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   r->achsDtemp.re = achs.re;
   r->achsDtemp.im = achs.im;
   std::vector<Cplx> asx(NE), cor(NE), a2(NE);
-  if ((parts & FF_PART_ASX) && sums_possible)
-    for (int b = 0; b < NV; ++b)
-      for (int iw = 0; iw < NE; ++iw) {
-        const Fact &fc = fasx[(size_t)b * NE + iw];
-        if (fc.f < 0) continue;
-        const Cplx v = interp(kasx[(size_t)b * NE + iw], fc, b);
-        asx[iw].add(p->occ * v.re, p->occ * v.im);
-      }
-  if ((parts & FF_PART_COR) && sums_possible)
-    for (int b = 0; b < NB; ++b)
-      for (int iw = 0; iw < NE; ++iw) {
-        const Fact &fc = fcor[(size_t)b * NE + iw];
-        if (fc.f < 0) continue;
-        const Cplx x = interp(kR, fc, b), y = interp(kA, fc, b);
-        Cplx v;
-        v.re = x.re - y.re;
-        v.im = x.im - y.im;
-        if (pcor[(size_t)b * NE + iw]) {
-          a2[iw].add(v.re, v.im);
-          if (b >= NV) cor[iw].add(v.re, v.im);
-        } else {
-          cor[iw].add(-0.5 * v.re, -0.5 * v.im);
+  // Blocks of evaluation frequencies on several host threads; each entry
+  // still adds its bands in order, so the sums are those of one thread.
+  constexpr int kAsmBlock = 1024;
+  auto assemble = [&](int blk) {
+    const int i0 = blk * kAsmBlock, i1 = std::min(NE, i0 + kAsmBlock);
+    if ((parts & FF_PART_ASX) && sums_possible)
+      for (int b = 0; b < NV; ++b)
+        for (int iw = i0; iw < i1; ++iw) {
+          const Fact &fc = fasx[(size_t)b * NE + iw];
+          if (fc.f < 0) continue;
+          const Cplx v = interp(kasx[(size_t)b * NE + iw], fc, b);
+          asx[iw].add(p->occ * v.re, p->occ * v.im);
         }
-      }
+    if ((parts & FF_PART_COR) && sums_possible)
+      for (int b = 0; b < NB; ++b)
+        for (int iw = i0; iw < i1; ++iw) {
+          const Fact &fc = fcor[(size_t)b * NE + iw];
+          if (fc.f < 0) continue;
+          const Cplx x = interp(kR, fc, b), y = interp(kA, fc, b);
+          Cplx v;
+          v.re = x.re - y.re;
+          v.im = x.im - y.im;
+          if (pcor[(size_t)b * NE + iw]) {
+            a2[iw].add(v.re, v.im);
+            if (b >= NV) cor[iw].add(v.re, v.im);
+          } else {
+            cor[iw].add(-0.5 * v.re, -0.5 * v.im);
+          }
+        }
+  };
+  parallel_ranges(&pc->pool, (NE + kAsmBlock - 1) / kAsmBlock,
+                  (long long)NB * NE, 20000, assemble);
   // the frequency-grid results (their stream may still be running)
   double asm_ms =
       std::chrono::duration<double, std::milli>(clk::now() - t_asm).count();

@@ -538,6 +538,7 @@ int gpp_destroy(gpp_handle h) {
   if (h->host_out) (void)hipHostFree(h->host_out);
   if (h->host_grid) (void)hipHostFree(h->host_grid);
   delete h->stager;
+  gpp::ff::free_plan_cache(h->ff_plan);
   if (h->ff_stream) (void)hipStreamDestroy(h->ff_stream);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

@@ -15,6 +15,11 @@
 #include "gpp_stage.h"
 
 namespace gpp {
+namespace ff {
+// FF host planning buffers of a handle (defined in ff_api.cpp).
+struct PlanCache;
+void free_plan_cache(PlanCache *c);
+}  // namespace ff
 
 // Device buffer owned by a handle, grown on demand.
 struct DevBuf {
@@ -112,5 +117,6 @@ struct gpp_handle_s {
   int batch_nrec = 0;
   std::string err;
   gpp::Stager *stager = nullptr;  // host-input copies (created on first use)
+  gpp::ff::PlanCache *ff_plan = nullptr;  // FF host planning buffers
   long long growths = 0;          // buffer growths (failure injection count)
 };

@@ -1,8 +1,9 @@
 """CPU-only sanitizer builds (tests/native/Makefile): the oracle under
 AddressSanitizer + UndefinedBehaviorSanitizer (with its OpenMP loops), and
 the host code that starts threads of its own -- the stager's copy pool
-(fortkernels-_amd/csrc/gpp_stage.cpp) and the input generator
-(gpp_inputs.cpp) -- under ThreadSanitizer (and ASan for the pool).  Each
+(fortkernels-_amd/csrc/gpp_stage.cpp), the input generator
+(gpp_inputs.cpp) and the FF planning's worker pool (host_pool.h) -- under
+ThreadSanitizer (and ASan for the pools).  Each
 driver must exit 0 with no sanitizer report.  The reference has real races
 (gppKer.cpp:67,97) and an out-of-bounds read (:288) that the oracle's
 restatement does not repeat."""
@@ -15,7 +16,8 @@ import pytest
 from conftest import ROOT
 
 NATIVE = ROOT / "tests" / "native"
-DRIVERS = ["oracle_asan", "copypool_tsan", "copypool_asan", "inputs_tsan"]
+DRIVERS = ["oracle_asan", "copypool_tsan", "copypool_asan", "inputs_tsan",
+           "hostpool_tsan", "hostpool_asan"]
 
 
 @pytest.fixture(scope="module")
