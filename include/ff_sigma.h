@@ -22,7 +22,9 @@
  * Layout as ffKernel.cpp (row-major, ig fastest): aqsm/aqsn [NB][NC],
  * I_epsR/I_epsA [nFreq][NG][NC], schDt_matrix [NB][nFreq]; complex numbers
  * are gpp_c128 {re, im}.  Conventions (return codes, handles, ownership) as
- * in gpp_sigma.h.
+ * in gpp_sigma.h.  The host planning and assembly of an FF call run on up
+ * to 7 worker threads of the handle besides the caller's, started by the
+ * handle's first FF call and joined by gpp_destroy.
  */
 #ifndef FF_SIGMA_H
 #define FF_SIGMA_H
