@@ -91,7 +91,13 @@ inline void parallel_ranges(HostPool *pool, int n, long long work,
     for (int i = 0; i < n; ++i) fn(i);
     return;
   }
-  const std::function<void(int)> f = fn;
+  std::function<void(int)> f;
+  try {
+    f = fn;  // may allocate
+  } catch (...) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
   pool->run(n, f);
 }
 
