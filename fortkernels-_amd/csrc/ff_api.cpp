@@ -24,6 +24,7 @@
 #include <cstring>
 #include <vector>
 
+#include "gpp_guard.h"
 #include "../../include/ff_sigma.h"
 #include "ff_internal.h"
 #include "gpp_handle.h"
@@ -320,7 +321,7 @@ void ff_default_options(ff_options *o) {
 }
 
 int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
-                ff_result *r) {
+                ff_result *r) try {
   if (!h) return GPP_EINVAL;
   if (!r) return set_err(h, GPP_EINVAL, "null result");
   h->err.clear();
@@ -367,6 +368,7 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   if (do_ach && (rc = launch_grid_terms(h, p, host, sem))) return rc;
 
   // ---- plan: which (kind, slice, band) sums are needed
+  gpp::simulated_host_alloc_failure();
   gpp::ff::PlanCache *pc = nullptr;
   if ((rc = plan_buffers(h, (size_t)2 * nF * NB, (size_t)NV * NE,
                          (size_t)NB * NE, (size_t)2 * nF, &pc)))
@@ -724,6 +726,8 @@ int ff_sigma_ex(gpp_handle h, const ff_problem *p, const ff_options *o,
   r->slice_terms = nslice_bands * (long long)per_slice;
   r->freq_terms = do_ach ? (long long)NB * nF * NE : 0;
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(h);
 }
 
 }  // extern "C"
@@ -757,7 +761,7 @@ int ff_achsDtemp_Kernel(int number_bands, int ngpown, int ncouls,
                         const int *inv_igp_index, const int *indinv,
                         const gpp_c128 *aqsntemp, const gpp_c128 *aqsmtemp,
                         const gpp_c128 *I_epsR_array, const double *vcoul,
-                        gpp_c128 *achsDtemp) {
+                        gpp_c128 *achsDtemp) try {
   if (!achsDtemp) return GPP_EINVAL;
   ff_problem p{};
   p.number_bands = number_bands;
@@ -775,6 +779,8 @@ int ff_achsDtemp_Kernel(int number_bands, int ngpown, int ncouls,
   if (rc) return rc;
   *achsDtemp = r.achsDtemp;  // :128 assigns
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 int ff_asxDtemp_Kernel(int nvband, int nfreqeval, int ncouls, int ngpown,
@@ -784,7 +790,7 @@ int ff_asxDtemp_Kernel(int nvband, int nfreqeval, int ncouls, int ngpown,
                        const int *indinv, const gpp_c128 *aqsmtemp,
                        const gpp_c128 *aqsntemp, const double *vcoul,
                        const gpp_c128 *I_epsR_array, gpp_c128 *ssxDittt,
-                       gpp_c128 *asxDtemp) {
+                       gpp_c128 *asxDtemp) try {
   (void)numThreads;
   (void)ssxDittt;
   if (!asxDtemp || nfreqeval < 0) return GPP_EINVAL;
@@ -813,6 +819,8 @@ int ff_asxDtemp_Kernel(int nvband, int nfreqeval, int ncouls, int ngpown,
   if (rc) return rc;
   add_into(asxDtemp, out);  // :151 adds
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 int ff_achDtemp_Kernel(int number_bands, int nvband, int nfreqeval,
@@ -821,7 +829,7 @@ int ff_achDtemp_Kernel(int number_bands, int nvband, int nfreqeval,
                        const double *pref, const double *dFreqGrid,
                        const gpp_c128 *dFreqBrd, const gpp_c128 *schDt_matrix,
                        gpp_c128 *schDi, gpp_c128 *schDi_cor, gpp_c128 *sch2Di,
-                       gpp_c128 *achDtemp) {
+                       gpp_c128 *achDtemp) try {
   (void)schDi;
   (void)schDi_cor;
   (void)sch2Di;
@@ -848,6 +856,8 @@ int ff_achDtemp_Kernel(int number_bands, int nvband, int nfreqeval,
   if (rc) return rc;
   add_into(achDtemp, out);  // :206 adds
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 int ff_achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval,
@@ -860,7 +870,7 @@ int ff_achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval,
                            const gpp_c128 *I_epsA_array, gpp_c128 *schDi_cor,
                            gpp_c128 *schDi_corb, gpp_c128 *sch2Di,
                            gpp_c128 *ach2Dtemp, gpp_c128 *achDtemp_cor,
-                           gpp_c128 *achDtemp_corb) {
+                           gpp_c128 *achDtemp_corb) try {
   (void)schDi_cor;
   (void)schDi_corb;
   (void)sch2Di;
@@ -893,6 +903,8 @@ int ff_achDtemp_cor_Kernel(int number_bands, int nvband, int nfreqeval,
   add_into(ach2Dtemp, a2);
   (void)achDtemp_corb;
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 }  // extern "C"

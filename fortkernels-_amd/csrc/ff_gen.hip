@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "gpp_guard.h"
 #include "../../include/ff_sigma.h"
 #include "ff_internal.h"
 #include "gpp_rng.h"
@@ -201,7 +202,7 @@ int ff_fill_inputs(int config, int semantics, unsigned long long seed,
                    gpp_c128 *aqsn, gpp_c128 *epsR, gpp_c128 *epsA,
                    double *vcoul, int *inv, int *indinv, double *ekq,
                    double *grid, double *pref, gpp_c128 *brd,
-                   gpp_c128 *schdt, double *scal) {
+                   gpp_c128 *schdt, double *scal) try {
   if (bad_shape(config, semantics, NB, NC, NG, nF)) return GPP_EINVAL;
   if (!aqsm || !aqsn || !epsR || !epsA || !vcoul || !inv || !indinv ||
       !ekq || !grid || !pref || !brd || !schdt || !scal)
@@ -238,6 +239,8 @@ int ff_fill_inputs(int config, int semantics, unsigned long long seed,
     }
   });
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 int ff_fill_inputs_device(int config, int semantics, unsigned long long seed,
@@ -245,7 +248,7 @@ int ff_fill_inputs_device(int config, int semantics, unsigned long long seed,
                           gpp_c128 *aqsn, gpp_c128 *epsR, gpp_c128 *epsA,
                           double *vcoul, int *inv, int *indinv, double *ekq,
                           double *grid, double *pref, gpp_c128 *brd,
-                          gpp_c128 *schdt, double *scal, void *stream) {
+                          gpp_c128 *schdt, double *scal, void *stream) try {
   if (bad_shape(config, semantics, NB, NC, NG, nF)) return GPP_EINVAL;
   if (!aqsm || !aqsn || !epsR || !epsA || !vcoul || !inv || !indinv ||
       !ekq || !grid || !pref || !brd || !schdt || !scal)
@@ -274,6 +277,8 @@ int ff_fill_inputs_device(int config, int semantics, unsigned long long seed,
     return GPP_EDEVICE;
   // The host tables above are freed on return: finish their copies first.
   return hipStreamSynchronize(s) == hipSuccess ? GPP_OK : GPP_EDEVICE;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 }  // extern "C"

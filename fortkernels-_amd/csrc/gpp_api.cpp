@@ -20,13 +20,16 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <mutex>
+#include <new>
 #include <string>
 #include <tuple>
 #include <vector>
 
 #include "../../include/gpp_sigma.h"
+#include "gpp_guard.h"
 #include "gpp_handle.h"
 
 using gpp::BatchItem;
@@ -67,6 +70,25 @@ int set_err(gpp_handle h, int code, const char *fmt, ...) {
   return code;
 }
 
+int exception_code(gpp_handle h) noexcept {
+  int code = GPP_EDEVICE;
+  const char *what = "internal error";
+  try {
+    throw;
+  } catch (const std::bad_alloc &) {
+    code = GPP_ENOMEM;
+    what = "host memory allocation failed";
+  } catch (const std::exception &e) {
+    what = e.what();
+  } catch (...) {
+  }
+  try {
+    set_err(h, code, "%s", what);
+  } catch (...) {
+  }
+  return code;
+}
+
 // Failure injection for tests: with GPP_FAIL_ALLOC_AFTER=N in the
 // environment, the N-th buffer growth of a handle (device buffers and the
 // pinned result buffer, counted from 1 per handle) fails as an
@@ -75,6 +97,15 @@ bool simulated_alloc_failure(gpp_handle h) {
   const char *v = std::getenv("GPP_FAIL_ALLOC_AFTER");
   const long long n = ++h->growths;
   return v && *v && n == std::atoll(v);
+}
+
+// Failure injection for tests: with GPP_FAIL_HOST_ALLOC set (non-empty),
+// the host planning of gpp_sigma_batch and ff_sigma_ex throws
+// std::bad_alloc as a failed host allocation would, which the entry
+// point's handler turns into GPP_ENOMEM.
+void simulated_host_alloc_failure() {
+  const char *v = std::getenv("GPP_FAIL_HOST_ALLOC");
+  if (v && *v) throw std::bad_alloc();
 }
 
 int ensure(gpp_handle h, DevBuf &b, size_t bytes) {
@@ -483,7 +514,7 @@ void gpp_default_options(gpp_options *o) {
   o->memory = GPP_MEM_HOST;
 }
 
-int gpp_create(gpp_handle *out, int device) {
+int gpp_create(gpp_handle *out, int device) try {
   if (!out) return GPP_EINVAL;
   *out = nullptr;
   gpp_handle h = new (std::nothrow) gpp_handle_s;
@@ -521,6 +552,8 @@ int gpp_create(gpp_handle *out, int device) {
   load_tune_file(h);
   *out = h;
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 int gpp_destroy(gpp_handle h) {
@@ -551,7 +584,7 @@ const char *gpp_last_error(gpp_handle h) {
 }
 
 int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
-                 gpp_result *r) {
+                 gpp_result *r) try {
   if (!h) return GPP_EINVAL;
   if (!r) return set_err(h, GPP_EINVAL, "null result");
   h->err.clear();
@@ -705,10 +738,12 @@ int gpp_sigma_ex(gpp_handle h, const gpp_problem *p, const gpp_options *o,
         return set_err(h, GPP_ENONFINITE, "non-finite result in slot %d", i);
   }
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(h);
 }
 
 int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *o,
-                 int *tile_ig, int *waves_per_block) {
+                 int *tile_ig, int *waves_per_block) try {
   if (!h) return GPP_EINVAL;
   if (!tile_ig || !waves_per_block)
     return set_err(h, GPP_EINVAL, "null output");
@@ -760,10 +795,12 @@ int gpp_autotune(gpp_handle h, const gpp_problem *p, const gpp_options *o,
   *tile_ig = best.k_ig;
   *waves_per_block = best.waves;
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(h);
 }
 
 int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
-                    const gpp_options *o, gpp_result *r) {
+                    const gpp_options *o, gpp_result *r) try {
   if (!h) return GPP_EINVAL;
   h->err.clear();
   if (n < 0 || (n > 0 && (!p || !r)) || !o)
@@ -851,6 +888,7 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   }
   h->batch_key.clear();
 
+  gpp::simulated_host_alloc_failure();
   std::vector<Prepared> prep;
   if ((rc = prepare(h, p, n, o, mixed, s, prep))) return rc;
 
@@ -1021,6 +1059,8 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
     h->batch_nrec = nrec + nach;
   }
   return finish_batch(h, p, n, o, want_ach, nrec + nach, s, r);
+} catch (...) {
+  return gpp::exception_code(h);
 }
 
 int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
@@ -1028,7 +1068,7 @@ int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
               const gpp_c128 *I_eps_array, const gpp_c128 *wtilde_array,
               const double *vcoul, const int *inv_igp_index,
               const int *indinv, const double *wx_array, gpp_c128 *achtemp,
-              gpp_c128 *asxtemp) {
+              gpp_c128 *asxtemp) try {
   if (!achtemp || !asxtemp) return GPP_EINVAL;
   gpp::DefaultHandle dh;
   if (dh.rc) return dh.rc;
@@ -1049,6 +1089,8 @@ int gpp_sigma(int number_bands, int nvband, int ncouls, int ngpown,
     asxtemp[i] = res.asxtemp[i];
   }
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
@@ -1058,7 +1100,7 @@ int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
                        const int *indinv, int number_bands,
                        double *achtemp_re0, double *achtemp_re1,
                        double *achtemp_re2, double *achtemp_im0,
-                       double *achtemp_im1, double *achtemp_im2) {
+                       double *achtemp_im1, double *achtemp_im2) try {
   if (!achtemp_re0 || !achtemp_re1 || !achtemp_re2 || !achtemp_im0 ||
       !achtemp_im1 || !achtemp_im2)
     return GPP_EINVAL;
@@ -1084,6 +1126,8 @@ int gpp_achtemp_kernel(int ncouls, int ngpown, const double *wx_array,
   *achtemp_im1 += res.achtemp[1].im;
   *achtemp_im2 += res.achtemp[2].im;
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
@@ -1092,7 +1136,7 @@ int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
                           const gpp_c128 *aqsmtemp, const gpp_c128 *aqsntemp,
                           const gpp_c128 *I_eps_array, int ncouls, int nvband,
                           int ngpown, const double *vcoul, int nstart,
-                          int nend, double occ, int number_bands) {
+                          int nend, double occ, int number_bands) try {
   if (!asxtemp || nstart < 0 || nend > 3 || nstart > nend) return GPP_EINVAL;
   gpp::DefaultHandle dh;
   if (dh.rc) return dh.rc;
@@ -1114,6 +1158,8 @@ int gpp_tillNvband_kernel(const int *inv_igp_index, const int *indinv,
     asxtemp[iw].im += res.asxtemp[iw].im * occ;
   }
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <vector>
 
+#include "gpp_guard.h"
 #include "../../include/gpp_sigma.h"
 #include "gpp_rng.h"
 
@@ -82,7 +83,7 @@ extern "C" int gpp_fill_inputs_device(
     int config, unsigned long long seed, int number_bands, int ncouls,
     int ngpown, int nodes_per_group, gpp_c128 *aqsmtemp, gpp_c128 *aqsntemp,
     gpp_c128 *I_eps_array, gpp_c128 *wtilde_array, double *vcoul,
-    int *inv_igp_index, int *indinv, double *wx_array, void *stream) {
+    int *inv_igp_index, int *indinv, double *wx_array, void *stream) try {
   const int NB = number_bands, NC = ncouls, NG = ngpown;
   const bool vlin = config & GPP_FILL_VCOUL_LINEAR;
   config &= ~GPP_FILL_VCOUL_LINEAR;
@@ -172,4 +173,6 @@ extern "C" int gpp_fill_inputs_device(
   GEN_OR_RET(hipStreamSynchronize(s));
 #undef GEN_OR_RET
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }

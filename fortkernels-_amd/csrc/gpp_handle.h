@@ -70,6 +70,10 @@ struct DefaultHandle {
   int ret(int code);
 };
 
+// Throws std::bad_alloc when GPP_FAIL_HOST_ALLOC is set (tests of the
+// entry points' exception handlers).
+GPP_INTERNAL void simulated_host_alloc_failure();
+
 // Validation shared by every entry point that takes a gpp_problem.
 GPP_INTERNAL int validate(gpp_handle h, const gpp_problem *p,
                           const gpp_options *o);

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -155,7 +156,7 @@ extern "C" int gpp_fill_inputs(int config, unsigned long long seed,
                                gpp_c128 *aqsntemp, gpp_c128 *I_eps_array,
                                gpp_c128 *wtilde_array, double *vcoul,
                                int *inv_igp_index, int *indinv,
-                               double *wx_array) {
+                               double *wx_array) try {
   const bool vlin = config & GPP_FILL_VCOUL_LINEAR;
   const int rc = fill_config(config & ~GPP_FILL_VCOUL_LINEAR, seed,
                              number_bands, ncouls, ngpown, nodes_per_group,
@@ -164,4 +165,8 @@ extern "C" int gpp_fill_inputs(int config, unsigned long long seed,
   if (rc == GPP_OK && vlin)
     for (int i = 0; i < ncouls; ++i) vcoul[i] = (double)i / ncouls;
   return rc;
+} catch (const std::bad_alloc &) {
+  return GPP_ENOMEM;  // no exception crosses the C ABI
+} catch (...) {
+  return GPP_EDEVICE;
 }

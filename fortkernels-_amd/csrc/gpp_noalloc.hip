@@ -22,6 +22,7 @@
 #include <cstddef>
 #include <mutex>
 
+#include "gpp_guard.h"
 #include "../../include/gpp_sigma.h"
 #include "gpp_handle.h"
 
@@ -69,7 +70,7 @@ int scratch_of(int dev, unsigned char **out) {
 }  // namespace
 
 extern "C" int gpp_sigma_noalloc(const gpp_problem *p, const gpp_options *o,
-                                 gpp_c128 *out) {
+                                 gpp_c128 *out) try {
   int rc = gpp::validate(nullptr, p, o);
   if (rc) return rc;
   if (!out) return gpp::set_err(nullptr, GPP_EINVAL, "null output buffer");
@@ -177,4 +178,6 @@ extern "C" int gpp_sigma_noalloc(const gpp_problem *p, const gpp_options *o,
   NOALLOC_HIP(hipMemcpyAsync(out + 6, dres, sizeof(gpp_c128),
                              hipMemcpyDeviceToDevice, s));
   return GPP_OK;
+} catch (...) {
+  return gpp::exception_code(nullptr);
 }
