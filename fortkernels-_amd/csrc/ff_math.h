@@ -30,13 +30,6 @@
 #define FF_HD
 #endif
 
-// True when `c` holds on any lane of the wave (on the host: c).  A branch on
-// it is uniform: one scalar compare, no exec-mask bookkeeping.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define FF_ANY_LANE(c) (__builtin_amdgcn_ballot_w64(c) != 0)
-#else
-#define FF_ANY_LANE(c) (c)
-#endif
 
 namespace gpp {
 namespace ff {
@@ -93,25 +86,48 @@ FF_HD inline double clamp_log_ratio(double t) {
 // passed through.
 struct IntervalLog {
   double l;  // log|a_{f-1}| (log_abs_nonzero)
-  bool z;    // a_{f-1} == 0
-  FF_HD void start(double a, const double *tab) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // The zero tests are kept as wave lane masks (scalar registers): one
+  // compare per point and no per-lane mask bookkeeping.
+  uint64_t zm;  // lanes with a_{f-1} == 0
+  __device__ void start(double a, const double *tab) {
+    l = log_abs_nonzero(a, tab);
+    zm = __builtin_amdgcn_ballot_w64(a == 0.0);
+  }
+  __device__ double step(double a, const double *tab) {
+    const double lr = log_abs_nonzero(a, tab);
+    double t = lr - l;
+    const uint64_t zr = __builtin_amdgcn_ballot_w64(a == 0.0);
+    // -kLogRatioLo < kLogRatioHi: |t| <= -kLogRatioLo is inside both bounds
+    const uint64_t ex =
+        __builtin_amdgcn_ballot_w64(!(fabs(t) <= -kLogRatioLo)) | zr | zm;
+    if (ex != 0) {
+      const uint64_t bit = 1ull << __lane_id();
+      const double te = clamp_log_ratio((zr & bit ? -INFINITY : lr) -
+                                        (zm & bit ? -INFINITY : l));
+      t = ex & bit ? te : t;
+    }
+    l = lr;
+    zm = zr;
+    return t;
+  }
+#else
+  bool z;  // a_{f-1} == 0
+  void start(double a, const double *tab) {
     l = log_abs_nonzero(a, tab);
     z = a == 0.0;
   }
-  FF_HD double step(double a, const double *tab) {
+  double step(double a, const double *tab) {
     const double lr = log_abs_nonzero(a, tab);
     const bool zr = a == 0.0;
     double t = lr - l;
-    // -kLogRatioLo < kLogRatioHi: |t| <= -kLogRatioLo is inside both bounds
-    const bool exact = !(fabs(t) <= -kLogRatioLo) | zr | z;
-    if (FF_ANY_LANE(exact)) {
-      const double te = clamp_log_ratio((zr ? -INFINITY : lr) - (z ? -INFINITY : l));
-      t = exact ? te : t;
-    }
+    if (!(fabs(t) <= -kLogRatioLo) || zr || z)
+      t = clamp_log_ratio((zr ? -INFINITY : lr) - (z ? -INFINITY : l));
     l = lr;
     z = zr;
     return t;
   }
+#endif
 };
 
 }  // namespace ff
