@@ -506,12 +506,12 @@ class Handle:
         return _batch_results(res, n)
 
     def autotune(self, inp: dict, precision="fp64", semantics="gppker",
-                 memory=None, stream=None) -> tuple:
+                 memory=None, stream=None, resonance_threshold=0.0) -> tuple:
         """Run the tile autotuner for this problem's shape class; returns
         (tile_ig, waves_per_block), also cached in the handle."""
         prob = make_problem(inp)
         opt = self._options(inp, precision, semantics, memory, 0, stream,
-                            0.0, 0, 0)
+                            resonance_threshold, 0, 0)
         k, w = C.c_int(), C.c_int()
         rc = self.lib.gpp_autotune(self.h, C.byref(prob), C.byref(opt),
                                    C.byref(k), C.byref(w))
