@@ -185,7 +185,15 @@ __device__ __forceinline__ void prepass_one(int i, const int *__restrict__ inv_i
     const float hi = (float)w;
     split[0] = hi;
     split[1] = (float)(w - (double)hi);
-    if (i % 3 == 2) split[2] = split[3] = 0.0f;
+    if (i % 3 == 2) {
+      // slot 6 of the row: nonzero when a frequency of the band is not
+      // finite (the mixed kernel then runs the band in FP64); slot 7: 0
+      const double w0 = wx_per_band ? wx_in[i - 2] : wx_in[0];
+      const double w1 = wx_per_band ? wx_in[i - 1] : wx_in[1];
+      const bool fin = isfinite(w0) && isfinite(w1) && isfinite(w);
+      split[2] = fin ? 0.0f : 1.0f;
+      split[3] = 0.0f;
+    }
   }
 }
 
@@ -481,11 +489,13 @@ __device__ __forceinline__ void unocc_fp64_block(const Prob &p,
         for (int iw = 0; iw < 3; ++iw) {
           const double dr = xs[iw] - w.x;
           const double d = fma(dr, dr, wy2);
-          double r = 1.0 / d;
-          if (MASK)
-            r = (d > kLimitTwo && (w.x * w.x + wy2) * r < kLimitOne) ? r : 0.0;
-          sr[iw] = (Br * dr - Bi * w.y) * r;
-          si[iw] = (Br * w.y + Bi * dr) * r;
+          const double r = 1.0 / d;
+          // the mask skips the term outright (a NaN term it drops adds
+          // nothing, as in the f90 loop)
+          const bool keep =
+              !MASK || (d > kLimitTwo && (w.x * w.x + wy2) * r < kLimitOne);
+          sr[iw] = keep ? (Br * dr - Bi * w.y) * r : 0.0;
+          si[iw] = keep ? (Br * w.y + Bi * dr) * r : 0.0;
         }
         s_r0 += sr[0];
         s_i0 += si[0];
@@ -634,9 +644,10 @@ __device__ __forceinline__ void mixed_fp64_run(const Prob &p, int n1, int my_igp
     for (int iw = 0; iw < 3; ++iw) {
       const double ddr = x[iw] - w.x;
       const double dd = fma(ddr, ddr, wy2);
-      double rr = 1.0 / dd;
-      if (MASK)
-        rr = (dd > kLimitTwo && (w.x * w.x + wy2) * rr < kLimitOne) ? rr : 0.0;
+      const double rr = 1.0 / dd;
+      // the mask skips the term outright (a NaN term it drops adds nothing)
+      if (MASK && !(dd > kLimitTwo && (w.x * w.x + wy2) * rr < kLimitOne))
+        continue;
       sr[iw] += (B_r * ddr - B_i * w.y) * rr;
       si[iw] += (B_r * w.y + B_i * ddr) * rr;
     }
@@ -675,6 +686,7 @@ __device__ __forceinline__ void unocc_mixed_block(
 
   float wrh[K], wrl[K], wi[K], wi2[K], Ar[K], Ai[K];
   float idm[K];  // MASK: 1 / the mask threshold max(limittwo, |wtilde|^2/limitone)
+  bool lane_nf = false;  // a wtilde of the run is not finite: runs in FP64
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int ig = ig0 + k;
@@ -682,6 +694,7 @@ __device__ __forceinline__ void unocc_mixed_block(
       const size_t off = (size_t)my_igp * NC + ig;
       const double2 w = ld(p.wtilde + off);
       const double2 e = ld(p.ieps + off);
+      lane_nf |= !(isfinite(w.x) && isfinite(w.y));
       wrh[k] = (float)w.x;
       wrl[k] = (float)(w.x - (double)wrh[k]);
       wi[k] = (float)w.y;
@@ -709,19 +722,20 @@ __device__ __forceinline__ void unocc_mixed_block(
   int qn = 0;  // entries waiting in this wave's list (wave-uniform)
   const auto aqsm_col = p.aqsmG + min(my_igp, NG - 1);
   // the prepass's FP32 split of the frequencies, per band
-  // [xh0 xl0 xh1 xl1 xh2 xl2 0 0]; the zero-padded FP32 aqsn rows
+  // [xh0 xl0 xh1 xl1 xh2 xl2 f 0] (f nonzero: a frequency is not finite);
+  // the zero-padded FP32 aqsn rows
   const auto xsplit = (decltype(p.aqsn32))(p.wx + 6 * (size_t)p.NB);
   const int ncp = padded_nc(NC);
   double2 am_next = (n1_begin < n1_end)
                         ? ld(aqsm_col + (size_t)n1_begin * NG)
                         : make_double2(0.0, 0.0);
-  float2 an_cur[K], xs_cur[3];
+  float2 an_cur[K], xs_cur[4];  // xs_cur[3].x: the band's non-finite flag
   if (n1_begin < n1_end) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
       an_cur[k] = ld(p.aqsn32 + ((size_t)n1_begin * ncp + ig0 + k));
 #pragma unroll
-    for (int iw = 0; iw < 3; ++iw) xs_cur[iw] = ld(xsplit + n1_begin * 4 + iw);
+    for (int iw = 0; iw < 4; ++iw) xs_cur[iw] = ld(xsplit + n1_begin * 4 + iw);
   }
 
   // Evaluates `cnt` (<= 64) entries from the top of the list in FP64, one
@@ -746,12 +760,12 @@ __device__ __forceinline__ void unocc_mixed_block(
     const double2 am = am_next;
     const int n1n = min(n1 + 1, n1_end - 1);
     am_next = ld(aqsm_col + (size_t)n1n * NG);
-    float2 an_nxt[K], xs_nxt[3];
+    float2 an_nxt[K], xs_nxt[4];
 #pragma unroll
     for (int k = 0; k < K; ++k)
       an_nxt[k] = ld(p.aqsn32 + ((size_t)n1n * ncp + ig0 + k));
 #pragma unroll
-    for (int iw = 0; iw < 3; ++iw) xs_nxt[iw] = ld(xsplit + n1n * 4 + iw);
+    for (int iw = 0; iw < 4; ++iw) xs_nxt[iw] = ld(xsplit + n1n * 4 + iw);
 
     float s_r[3] = {0.f, 0.f, 0.f}, s_i[3] = {0.f, 0.f, 0.f};
     float c_r[3] = {0.f, 0.f, 0.f}, c_i[3] = {0.f, 0.f, 0.f};  // KAHAN
@@ -803,7 +817,15 @@ __device__ __forceinline__ void unocc_mixed_block(
         si[iw] -= (double)c_i[iw];
       }
     }
-    const bool near = !(dmin >= thresh) || (MASK && !(bmin >= kMaskBand));
+    // A run with a non-finite wtilde or frequency goes to FP64 too, whose
+    // per-term form skips the terms the bgw mask drops (in FP32 a dropped
+    // NaN term would still add 0 * NaN).  Only valid lanes: an invalid
+    // lane's FP64 run would read past row NG.
+    // (the flag tested as an integer: a scalar compare)
+    const bool band_nf = __float_as_int(xs_cur[3].x) != 0;
+    const bool near =
+        valid && (lane_nf || band_nf || !(dmin >= thresh) ||
+                  (MASK && !(bmin >= kMaskBand)));
     const unsigned long long m = __ballot(near);
     if (m) {  // wave-uniform
       if (near) {
@@ -836,7 +858,7 @@ __device__ __forceinline__ void unocc_mixed_block(
 #pragma unroll
     for (int k = 0; k < K; ++k) an_cur[k] = an_nxt[k];
 #pragma unroll
-    for (int iw = 0; iw < 3; ++iw) xs_cur[iw] = xs_nxt[iw];
+    for (int iw = 0; iw < 4; ++iw) xs_cur[iw] = xs_nxt[iw];
   }
   while (qn > 0) drain(min(qn, kWave));
 

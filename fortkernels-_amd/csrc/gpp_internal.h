@@ -46,7 +46,9 @@ struct DevProblem {
   const double *wx;       // [NB][3], clamped, band-resolved; then [NB][3]
                           // wx^2 / |wx^2|^2 (occupied solver, gppker); then
                           // [NB][8] floats: wx split into FP32 hi + lo per
-                          // frequency and 2 zeros (mixed precision).
+                          // frequency, a flag (nonzero when a frequency of
+                          // the band is not finite) and a zero (mixed
+                          // precision).
                           // kWxDoubles per band.
   const float2 *aqsn32;   // [NB][padded_nc(NC)] FP32 copy, zero padded
                           // (mixed precision only)

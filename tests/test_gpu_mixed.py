@@ -162,3 +162,63 @@ def test_exact_resonance_nan_in_its_frequency_only(handle):
     assert np.isnan(got["achtemp"][1])
     assert np.all(np.isfinite(got["achtemp"][[0, 2]]))
     assert got["fp64_fallback_terms"] > 0
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_nan_frequency_in_its_entry_only(handle, sem):
+    """A NaN frequency of one band (band-resolved wx): the FP32 path forms
+    the three frequencies' reciprocals together, so the run goes to FP64 and
+    the NaN stays where the oracle has it (its own entry under gppker; the
+    bgw mask drops the term, as the f90 comparisons do); the other entries
+    match the oracle."""
+    inp = gppk.fill_inputs(3, 6, 256, 4, nvband=1, seed=9)
+    wx = inp["wx_array"].copy()
+    wx[3, 1] = np.nan
+    inp = dict(inp, wx_array=wx)
+    ref = orc.sigma(inp, sem)
+    got = handle.sigma(inp, precision="mixed", semantics=sem)
+    nan = np.isnan(ref["achtemp"])
+    assert nan[1] == (sem == "gppker") and not nan[0] and not nan[2]
+    assert np.array_equal(np.isnan(got["achtemp"]), nan)
+    keep = ~nan
+    assert rel_err(got["achtemp"][keep], ref["achtemp"][keep]) <= MIXED_BOUND[sem]
+    assert got["fp64_fallback_terms"] > 0
+    # the FP64 kernel's per-term fallback treats the band the same way
+    f64 = handle.sigma(inp, precision="fp64", semantics=sem)
+    assert np.array_equal(np.isnan(f64["achtemp"]), nan)
+    assert rel_err(f64["achtemp"][keep], ref["achtemp"][keep]) <= FP64_TOL
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+def test_nan_wtilde(handle, sem):
+    """A NaN wtilde: its lane's runs go to FP64, where (as in the oracle)
+    the bgw mask drops its terms and gppker keeps them (NaN in every
+    entry); the FP64 kernel's fallback agrees."""
+    inp = gppk.fill_inputs(3, 6, 256, 4, nvband=1, seed=10)
+    w = inp["wtilde_array"].copy()
+    w[5, 17] = complex(np.nan, 0.02)
+    inp = dict(inp, wtilde_array=w)
+    ref = orc.sigma(inp, sem)
+    nan = np.isnan(ref["achtemp"])
+    assert nan.all() == (sem == "gppker") and (sem == "gppker" or not nan.any())
+    for prec, tol in (("mixed", MIXED_BOUND[sem]), ("fp64", FP64_TOL)):
+        got = handle.sigma(inp, precision=prec, semantics=sem)
+        assert np.array_equal(np.isnan(got["achtemp"]), nan), prec
+        if not nan.any():
+            assert rel_err(got["achtemp"], ref["achtemp"]) <= tol, prec
+
+
+def test_large_values_stay_within_the_bound(handle):
+    """wtilde components and frequencies of 1e5 and more: FP32 terms of
+    |wdiff|^2 up to ~1e13 stay finite and within the mixed bound."""
+    inp = gppk.fill_inputs(3, 7, 300, 3, nvband=2, seed=11)
+    w = inp["wtilde_array"].copy()
+    w[1, 7] = 3e5 + 0.01j
+    w[2, 8] = 1.5 + 2e6j
+    wx = inp["wx_array"].copy()
+    wx[4, :] = [4e5, 4e5 + 1.0, 4e5 + 2.0]
+    inp = dict(inp, wtilde_array=w, wx_array=wx)
+    ref = orc.sigma(inp)
+    got = handle.sigma(inp, precision="mixed")
+    assert np.all(np.isfinite(got["achtemp"]))
+    assert entry_err(got, ref) <= MIXED_BOUND["gppker"]
