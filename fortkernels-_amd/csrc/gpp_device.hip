@@ -879,8 +879,14 @@ __device__ __forceinline__ void unocc_mixed_block(
   }
 }
 
+// The unmasked, log-free variants with K <= 8 are compiled with a minimum of
+// 3 waves per SIMD (K = 8 uses 124 VGPRs either way, i.e. 4 waves): the
+// schedule the compiler then picks took the canonical launch from 24.05 to
+// 23.57 ms (same box, same bits).  The other variants need more registers.
 template <int K, bool MASK, bool N1LOG, bool KAHAN>
-__global__ __launch_bounds__(256) void unocc_mixed_kernel(
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MASK || N1LOG || K > 8 ? 1 : 3)))
+void unocc_mixed_kernel(
     DevProblem p, int n1_begin, int n1_end, int grp_lo, int grp_cnt,
     double *__restrict__ partials, double *__restrict__ n1log, float thresh,
     unsigned long long *__restrict__ fallback_count) {
