@@ -100,6 +100,15 @@ hipError_t launch_unocc(const DevProblem &p, int n1_begin, int n1_end,
                         double res_thresh, double *partials, double *n1log,
                         unsigned long long *fallback_count, hipStream_t s,
                         int grp_lo = 0, int grp_cnt = -1);
+// The unmasked mixed-precision kernel of launch_unocc (gpp_mixed.hip): tile
+// k_ig, `waves` waves per block, a grid of `grid` blocks over G' groups
+// [grp_lo, grp_lo + grp_cnt).
+hipError_t launch_unocc_mixed(const DevProblem &p, int n1_begin, int n1_end,
+                              int k_ig, bool kahan, int waves,
+                              double res_thresh, double *partials,
+                              double *n1log,
+                              unsigned long long *fallback_count, int grp_lo,
+                              int grp_cnt, int grid, hipStream_t s);
 // Band-sum-first variant (wx shared by all bands): one block per 64 G'
 // columns x 64 G-vectors; same record and group conventions as
 // launch_unocc.
