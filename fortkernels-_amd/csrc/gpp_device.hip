@@ -709,9 +709,12 @@ __global__ __launch_bounds__(kOccThreads) void occ_kernel(
 //                 wtilde*I_eps * conj(wdiff_iw) / |wdiff_iw|^2
 //   P[my_igp][ig] = sum_{n1} conj(aqsm[n1][igp]) * aqsn[n1][ig]
 // P is a complex matrix product over the bands (K = n1_end - n1_begin),
-// formed on the FP64 matrix cores as four real products:
-//   Re P = Ar^T Br + Ai^T Bi,  Im P = Ar^T Bi - Ai^T Br
-// (A = aqsmG [n1][my_igp], B = aqsn [n1][ig]).  A block owns 64 G' columns
+// formed on the FP64 matrix cores as three real products (the 3M method):
+//   T1 = Ar^T Br,  T2 = Ai^T Bi,  T3 = (Ar - Ai)^T (Br + Bi)
+//   Re P = T1 + T2,  Im P = T3 - T1 + T2
+// (A = aqsmG [n1][my_igp], B = aqsn [n1][ig]): a quarter fewer MFMAs than
+// the four products Ar^T Br + Ai^T Bi, Ar^T Bi - Ai^T Br (2.68 against
+// 3.42 ms at 512/2/32768/20), the two operand sums formed in registers.  A block owns 64 G' columns
 // (one group, as the other kernels) by 64 G-vectors; each of its four waves
 // a 32 x 32 quarter: 2 x 2 tiles of v_mfma_f64_16x16x4 (16 x 16 outputs, 4
 // bands per step).  That leaves room for 3 waves per SIMD (2 waves of
@@ -736,29 +739,36 @@ __device__ __forceinline__ dbl4 mfma4(double a, double b, dbl4 c) {
 }
 
 // One step of 4 bands: a[mb] = A[n1][row tile mb], b[nb] = B[n1][col tile nb]
-// for this lane's (band, row/column) slot.
-__device__ __forceinline__ void gemm_step(const double2 (&a)[2],
-                                          const double2 (&b)[kGemmNB],
-                                          dbl4 (&pr)[2][kGemmNB],
-                                          dbl4 (&pi)[2][kGemmNB]) {
-  // First all accumulators once, then all again, so no MFMA waits on the
-  // previous one's result.
+// for this lane's (band, row/column) slot.  Each of the three products runs
+// over all tiles before the next, so no MFMA waits on the previous one's
+// result.  The third product takes (Ar - Ai)(Br + Bi) for conj(A) B (CONJ)
+// and (Ar + Ai)(Br + Bi) for A B.
+template <int NB, bool CONJ>
+__device__ __forceinline__ void mfma3_step(const double2 (&a)[2],
+                                           const double2 (&b)[NB],
+                                           dbl4 (&t1)[2][NB], dbl4 (&t2)[2][NB],
+                                           dbl4 (&t3)[2][NB]) {
+  double as[2], bs[NB];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+    as[mb] = CONJ ? a[mb].x - a[mb].y : a[mb].x + a[mb].y;
+#pragma unroll
+  for (int nb = 0; nb < NB; ++nb) bs[nb] = b[nb].x + b[nb].y;
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-    for (int nb = 0; nb < kGemmNB; ++nb) {
-      pr[mb][nb] = mfma4(a[mb].x, b[nb].x, pr[mb][nb]);
-      pi[mb][nb] = mfma4(a[mb].x, b[nb].y, pi[mb][nb]);
-    }
+    for (int nb = 0; nb < NB; ++nb)
+      t1[mb][nb] = mfma4(a[mb].x, b[nb].x, t1[mb][nb]);
 #pragma unroll
-  for (int mb = 0; mb < 2; ++mb) {
-    const double nai = -a[mb].y;
+  for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
-    for (int nb = 0; nb < kGemmNB; ++nb) {
-      pr[mb][nb] = mfma4(a[mb].y, b[nb].y, pr[mb][nb]);
-      pi[mb][nb] = mfma4(nai, b[nb].x, pi[mb][nb]);
-    }
-  }
+    for (int nb = 0; nb < NB; ++nb)
+      t2[mb][nb] = mfma4(a[mb].y, b[nb].y, t2[mb][nb]);
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+      t3[mb][nb] = mfma4(as[mb], bs[nb], t3[mb][nb]);
 }
 
 template <bool MASK>
@@ -779,13 +789,14 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
   const int c0 = ig_tile * kGemmN + (wave >> 1) * 16 * kGemmNB;
   const int li = lane & 15, lk = lane >> 4;
 
-  dbl4 pr[2][kGemmNB], pi[2][kGemmNB];
+  dbl4 t1[2][kGemmNB], t2[2][kGemmNB], t3[2][kGemmNB];
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
     for (int nb = 0; nb < kGemmNB; ++nb) {
-      pr[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
-      pi[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      t1[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      t2[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      t3[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
 
   // Bands go through LDS kGemmStage at a time, double-buffered.  Thread
@@ -844,11 +855,21 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
 #pragma unroll
       for (int nb = 0; nb < kGemmNB; ++nb)
         bv[nb] = Bs[buf][kr][wn + 16 * nb + li];
-      gemm_step(a, bv, pr, pi);
+      mfma3_step<kGemmNB, true>(a, bv, t1, t2, t3);
     }
     if (more) lstore(buf ^ 1);
     __syncthreads();
   }
+  // P = conj(A)^T B from the three products; the registers of T1..T3 are
+  // free again for the epilogue.
+  dbl4 pr[2][kGemmNB], pi[2][kGemmNB];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < kGemmNB; ++nb) {
+      pr[mb][nb] = t1[mb][nb] + t2[mb][nb];
+      pi[mb][nb] = (t3[mb][nb] - t1[mb][nb]) + t2[mb][nb];
+    }
 
   // Epilogue.  Output (i, j) of a 16 x 16 tile sits in lane 16*(i%4) + j,
   // element i/4 (operands: A[i][k] in lane 16k + i, B[k][j] in lane
@@ -923,7 +944,8 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
 // with S2 = wtilde*I_eps*conj(wdiff_2)/|wdiff_2|^2, the third frequency's
 // factor of the sum (masked as the bgw semantics masks it).  Q is a complex
 // matrix product over the G-vectors, formed on the FP64 matrix cores like P
-// (Re Q = Ar Br - Ai Bi, Im Q = Ar Bi + Ai Br).  A block owns 64 bands by
+// with three real products (T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi);
+// Re Q = T1 - T2, Im Q = T3 - T1 - T2).  A block owns 64 bands by
 // 64 G' columns (one group) and one chunk of kLogChunk G-vectors; its four
 // waves hold 32 x 32 quarters.  Rows of aqsn and of S2 (computed from
 // wtilde and I_eps while staging) go through LDS kLogStage G-vectors at a
@@ -933,27 +955,6 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
 constexpr int kLogM = 64;        // bands per block
 constexpr int kLogStage = 8;     // G-vectors per LDS stage
 constexpr int kLogChunk = 2048;  // G-vectors per block
-
-__device__ __forceinline__ void log_step(const double2 (&a)[2],
-                                         const double2 (&b)[2],
-                                         dbl4 (&qr)[2][2], dbl4 (&qi)[2][2]) {
-#pragma unroll
-  for (int mb = 0; mb < 2; ++mb)
-#pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-      qr[mb][nb] = mfma4(a[mb].x, b[nb].x, qr[mb][nb]);
-      qi[mb][nb] = mfma4(a[mb].x, b[nb].y, qi[mb][nb]);
-    }
-#pragma unroll
-  for (int mb = 0; mb < 2; ++mb) {
-    const double nai = -a[mb].y;
-#pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-      qr[mb][nb] = mfma4(nai, b[nb].y, qr[mb][nb]);
-      qi[mb][nb] = mfma4(a[mb].y, b[nb].x, qi[mb][nb]);
-    }
-  }
-}
 
 template <bool MASK>
 __global__ __launch_bounds__(kGemmThreads) __attribute__((
@@ -973,13 +974,14 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
   const int li = lane & 15, lk = lane >> 4;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
 
-  dbl4 qr[2][2], qi[2][2];
+  dbl4 t1[2][2], t2[2][2], t3[2][2];
 #pragma unroll
   for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
-      qr[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
-      qi[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      t1[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      t2[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
+      t3[mb][nb] = dbl4{0.0, 0.0, 0.0, 0.0};
     }
 
   // Staging: thread t copies row t/4, G-vectors 2*(t%4) and 2*(t%4)+1 of
@@ -1051,11 +1053,19 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
       for (int mb = 0; mb < 2; ++mb) a[mb] = As[buf][wm + 16 * mb + li][kr];
 #pragma unroll
       for (int nb = 0; nb < 2; ++nb) b[nb] = Bs[buf][wn + 16 * nb + li][kr];
-      log_step(a, b, qr, qi);
+      mfma3_step<2, false>(a, b, t1, t2, t3);
     }
     if (more) lstore(buf ^ 1);
     __syncthreads();
   }
+  dbl4 qr[2][2], qi[2][2];
+#pragma unroll
+  for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      qr[mb][nb] = t1[mb][nb] - t2[mb][nb];
+      qi[mb][nb] = (t3[mb][nb] - t1[mb][nb]) - t2[mb][nb];
+    }
 
   // Epilogue: row i (band) of a tile sits in lanes 16*(i%4) + j, element
   // i/4; the 16 lanes of a row hold its 16 G' columns.
