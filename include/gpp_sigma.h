@@ -64,8 +64,10 @@ typedef struct { double re, im; } gpp_c128;
                                 wave work list                            */
 #define GPP_PREC_FP64_FACTORED 2 /* FP64 with the band sum taken first:
                                 P[my_igp][ig] = sum_n1 conj(aqsm[n1][igp]) *
-                                aqsn[n1][ig] on the FP64 matrix cores, then
-                                sum P * delw * I_eps; the occupied bands'
+                                aqsn[n1][ig] on the FP64 matrix cores
+                                (three real products per complex one; the
+                                error is bounded normwise, as for four),
+                                then sum P * delw * I_eps; the occupied bands'
                                 products are likewise summed before their
                                 solver; acht_n1_loc by a second product
                                 over the G-vectors.  Needs wx shared by
