@@ -244,6 +244,9 @@ __device__ __forceinline__ void unocc_fp64_block(const Prob &p,
   // between the bands; for larger K (or the masked variant's extra live
   // values) the registers do not hold two runs.
   constexpr bool kAhead = K <= 8 && !MASK && !N1LOG;
+  __shared__ double log_slot[N1LOG ? 2 : 1][kBandLogChunk][4][2];
+  const BandLog blog{log_slot, n1log, (size_t)b * (n1_end - n1_begin),
+                     n1_end - n1_begin};
   double2 an_cur[K];
   double x_cur[3] = {0.0, 0.0, 0.0};
   if (kAhead && n1_begin < n1_end) {
@@ -397,9 +400,8 @@ __device__ __forceinline__ void unocc_fp64_block(const Prob &p,
     acc_r[2] = fma(am.x, s_r2, fma(am.y, s_i2, acc_r[2]));
     acc_i[2] = fma(am.x, s_i2, fma(-am.y, s_r2, acc_i[2]));
     if (N1LOG)  // acht_n1_loc[n1] += sch_array[2] * vcoul[igp]
-      n1log_store(wgt * fma(am.x, s_r2, am.y * s_i2),
-                  wgt * fma(am.x, s_i2, -am.y * s_r2), n1log,
-                  (size_t)b * (n1_end - n1_begin) + (n1 - n1_begin));
+      blog.put(n1 - n1_begin, wgt * fma(am.x, s_r2, am.y * s_i2),
+               wgt * fma(am.x, s_i2, -am.y * s_r2));
     if (kAhead) {
 #pragma unroll
       for (int k = 0; k < K; ++k) an_cur[k] = an_nxt[k];

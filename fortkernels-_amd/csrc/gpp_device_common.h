@@ -126,6 +126,50 @@ __device__ __forceinline__ void n1log_store(double re, double im, double *dst,
   __syncthreads();
 }
 
+// Per-band log of a band loop: the same sums as n1log_store (lanes by
+// butterfly, then the block's waves in order) with one barrier per
+// kBandLogChunk bands instead of two per band.  Each wave's lane 0 puts its
+// wave sum for band j into an LDS slot; at the end of every chunk of
+// kBandLogChunk bands (and after the last band) the block waits once and
+// threads 0 .. chunk-1 add one band's slots over the waves and store it at
+// dst[2 * (base + j)].  The slots are double-buffered by chunk: a chunk's
+// slots are written again two chunks later, after the barrier that ends
+// the chunk in between, by which time every thread has stored its band.
+constexpr int kBandLogChunk = 64;
+struct BandLog {
+  double (*slot)[kBandLogChunk][4][2];  // [2][band in chunk][wave][re, im]
+  double *dst;
+  size_t base;
+  int nbands;
+
+  // Called by every thread of the block for band j = 0 .. nbands-1 in order.
+  __device__ __forceinline__ void put(int j, double re, double im) const {
+    re = wave_sum(re);
+    im = wave_sum(im);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int c = j / kBandLogChunk, i = j % kBandLogChunk;
+    if (lane == 0) {
+      slot[c & 1][i][wave][0] = re;
+      slot[c & 1][i][wave][1] = im;
+    }
+    if (i == kBandLogChunk - 1 || j == nbands - 1) {  // block-uniform
+      __syncthreads();
+      if ((int)threadIdx.x <= i) {
+        const int nw = blockDim.x / kWave;
+        double sr = 0.0, si = 0.0;
+        for (int w = 0; w < nw; ++w) {
+          sr += slot[c & 1][threadIdx.x][w][0];
+          si += slot[c & 1][threadIdx.x][w][1];
+        }
+        const size_t k = base + (size_t)c * kBandLogChunk + threadIdx.x;
+        dst[2 * k] = sr;
+        dst[2 * k + 1] = si;
+      }
+    }
+  }
+};
+
 // Order in which a launch over grp_cnt G' groups by T = gridDim.x /
 // grp_cnt G-vector tiles runs its tiles.  Blocks are handed to the XCDs
 // round-robin (block x runs on XCD x % 8) and each XCD has its own L2, so
@@ -255,6 +299,9 @@ __device__ __forceinline__ void unocc_mixed_block(
     unsigned long long *__restrict__ fallback_count) {
   __shared__ unsigned queue_all[4][kMixQueue];
   __shared__ double wgt_all[4][kWave];
+  __shared__ double log_slot[N1LOG ? 2 : 1][kBandLogChunk][4][2];
+  const BandLog blog{log_slot, n1log, (size_t)b * (n1_end - n1_begin),
+                     n1_end - n1_begin};
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = uniform(threadIdx.x / kWave);
   const int W = blockDim.x / kWave;
@@ -438,8 +485,7 @@ __device__ __forceinline__ void unocc_mixed_block(
       while (qn > 0) drain(min(qn, kWave));
       const double lr = fma(wr, sr[2], wi_ * si[2]) + (acc_r[2] - c2r);
       const double li = fma(wr, si[2], -wi_ * sr[2]) + (acc_i[2] - c2i);
-      n1log_store(lr, li, n1log,
-                  (size_t)b * (n1_end - n1_begin) + (n1 - n1_begin));
+      blog.put(n1 - n1_begin, lr, li);
     } else if (qn >= kWave) {
       drain(kWave);
     }
