@@ -244,9 +244,12 @@ __device__ __forceinline__ void unocc_fp64_block(const Prob &p,
   // between the bands; for larger K (or the masked variant's extra live
   // values) the registers do not hold two runs.
   constexpr bool kAhead = K <= 8 && !MASK && !N1LOG;
+  // per-band log: four bands per wave-level sum (16 KB of LDS per block)
   __shared__ double log_slot[N1LOG ? 2 : 1][kBandLogChunk][4][2];
-  const BandLog blog{log_slot, n1log, (size_t)b * (n1_end - n1_begin),
-                     n1_end - n1_begin};
+  __shared__ double2 log_stage[N1LOG ? 4 : 1][N1LOG ? 4 : 1][kWave];
+  const BandLog<N1LOG ? 4 : 1> blog{log_slot, log_stage[N1LOG ? wave : 0],
+                                    n1log, (size_t)b * (n1_end - n1_begin),
+                                    n1_end - n1_begin};
   double2 an_cur[K];
   double x_cur[3] = {0.0, 0.0, 0.0};
   if (kAhead && n1_begin < n1_end) {

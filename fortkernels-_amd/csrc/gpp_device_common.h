@@ -126,32 +126,69 @@ __device__ __forceinline__ void n1log_store(double re, double im, double *dst,
   __syncthreads();
 }
 
-// Per-band log of a band loop: the same sums as n1log_store (lanes by
-// butterfly, then the block's waves in order) with one barrier per
-// kBandLogChunk bands instead of two per band.  Each wave's lane 0 puts its
-// wave sum for band j into an LDS slot; at the end of every chunk of
+// Per-band log of a band loop: the sum of one complex value per lane over
+// the block for every band, stored at dst[2 * (base + j)], with one barrier
+// per kBandLogChunk bands instead of two per band (n1log_store).  Each wave
+// puts its sum for band j into an LDS slot; at the end of every chunk of
 // kBandLogChunk bands (and after the last band) the block waits once and
-// threads 0 .. chunk-1 add one band's slots over the waves and store it at
-// dst[2 * (base + j)].  The slots are double-buffered by chunk: a chunk's
-// slots are written again two chunks later, after the barrier that ends
-// the chunk in between, by which time every thread has stored its band.
+// threads 0 .. chunk-1 add one band's slots over the waves, in wave order,
+// and store it.  The slots are double-buffered by chunk: a chunk's slots
+// are written again two chunks later, after the barrier that ends the
+// chunk in between, by which time every thread has stored its band.
+// A wave's sum: G = 1, a butterfly over the lanes per band (as
+// n1log_store); G > 1, the values of G bands are staged in LDS and summed
+// together, kWave / G lanes per band each adding G entries and then a
+// butterfly over those lanes: about 10 instructions per band for G = 4
+// instead of about 50, but G x 1 KB of LDS per wave.  The lanes of a wave
+// execute their LDS accesses in order, so no fence is needed between the
+// staging writes and the reads.
 constexpr int kBandLogChunk = 64;
+template <int G>
 struct BandLog {
+  static_assert(kBandLogChunk % G == 0 && kWave % G == 0, "group size");
   double (*slot)[kBandLogChunk][4][2];  // [2][band in chunk][wave][re, im]
+  double2 (*stage)[kWave];              // G > 1: this wave's [G][kWave]
   double *dst;
   size_t base;
   int nbands;
 
   // Called by every thread of the block for band j = 0 .. nbands-1 in order.
   __device__ __forceinline__ void put(int j, double re, double im) const {
-    re = wave_sum(re);
-    im = wave_sum(im);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int c = j / kBandLogChunk, i = j % kBandLogChunk;
-    if (lane == 0) {
-      slot[c & 1][i][wave][0] = re;
-      slot[c & 1][i][wave][1] = im;
+    if constexpr (G == 1) {
+      re = wave_sum(re);
+      im = wave_sum(im);
+      if (lane == 0) {
+        slot[c & 1][i][wave][0] = re;
+        slot[c & 1][i][wave][1] = im;
+      }
+    } else {
+      const int g = j % G;
+      stage[g][lane] = make_double2(re, im);
+      if (g == G - 1 || j == nbands - 1) {  // wave-uniform
+        constexpr int L = kWave / G;  // lanes per band
+        const int band = lane / L, part = lane % L;
+        double sr = 0.0, si = 0.0;
+        if (band <= g) {
+#pragma unroll
+          for (int q = 0; q < G; ++q) {
+            const double2 v = stage[band][part * G + q];
+            sr += v.x;
+            si += v.y;
+          }
+        }
+#pragma unroll
+        for (int m = L / 2; m >= 1; m >>= 1) {
+          sr += __shfl_xor(sr, m, kWave);
+          si += __shfl_xor(si, m, kWave);
+        }
+        if (part == 0 && band <= g) {
+          slot[c & 1][i - g + band][wave][0] = sr;
+          slot[c & 1][i - g + band][wave][1] = si;
+        }
+      }
     }
     if (i == kBandLogChunk - 1 || j == nbands - 1) {  // block-uniform
       __syncthreads();
@@ -300,8 +337,8 @@ __device__ __forceinline__ void unocc_mixed_block(
   __shared__ unsigned queue_all[4][kMixQueue];
   __shared__ double wgt_all[4][kWave];
   __shared__ double log_slot[N1LOG ? 2 : 1][kBandLogChunk][4][2];
-  const BandLog blog{log_slot, n1log, (size_t)b * (n1_end - n1_begin),
-                     n1_end - n1_begin};
+  const BandLog<1> blog{log_slot, nullptr, n1log,
+                        (size_t)b * (n1_end - n1_begin), n1_end - n1_begin};
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = uniform(threadIdx.x / kWave);
   const int W = blockDim.x / kWave;
