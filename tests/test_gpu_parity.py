@@ -15,6 +15,7 @@ from test_oracle_cpu import (achstemp_expected, achstemp_problem, closed_form,
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-11
+ALL_FP64 = 1e30  # a resonance threshold above every |wdiff|^2: every run in FP64
 
 
 def check(got, ref, tol=FP64_TOL, keys=("achtemp", "asxtemp")):
@@ -281,6 +282,29 @@ def test_acht_n1_loc(handle, sem):
     assert e <= FP64_TOL, e
     # the per-band log sums to achtemp[2]
     assert rel_err(np.sum(got["acht_n1_loc"]), got["achtemp"][2]) <= 1e-12
+
+
+@pytest.mark.parametrize("sem", ["gppker", "bgw"])
+@pytest.mark.parametrize("waves", [1, 2, 4])
+def test_acht_n1_loc_many_bands(handle, sem, waves):
+    """201 non-occupied bands: the per-term kernels add the block's waves
+    once per 64 bands (four chunks, the last one partial) and the FP64
+    kernel sums four bands per wave at a time (the last group partial);
+    every band against the oracle, FP64 and the mixed path with all runs
+    through its FP64 work list, at 1, 2 and 4 waves per block."""
+    say("acht_n1_loc many bands", sem, waves)
+    inp = gppk.fill_inputs(3, 203, 256, 8, nvband=2, seed=23)
+    ref = orc.sigma(inp, sem, want_acht_n1=True)
+    for prec, kw in (("fp64", {}), ("mixed", {"resonance_threshold": ALL_FP64})):
+        got = handle.sigma(inp, precision=prec, semantics=sem,
+                           waves_per_block=waves, acht_n1_loc=True, **kw)
+        check(got, ref)
+        e = rel_err(got["acht_n1_loc"], ref["acht_n1_loc"])
+        assert e <= FP64_TOL, (prec, e)
+        # each band on its own, so a band landing in another's slot fails
+        for n1 in (0, 1, 2, 65, 66, 129, 130, 193, 200, 202):
+            assert abs(got["acht_n1_loc"][n1] - ref["acht_n1_loc"][n1]) <= \
+                FP64_TOL * abs(ref["acht_n1_loc"][n1]), (prec, n1)
 
 
 def test_acht_n1_loc_mixed(handle):
