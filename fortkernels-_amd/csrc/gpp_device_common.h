@@ -336,9 +336,14 @@ __device__ __forceinline__ void unocc_mixed_block(
     unsigned long long *__restrict__ fallback_count) {
   __shared__ unsigned queue_all[4][kMixQueue];
   __shared__ double wgt_all[4][kWave];
+  // per-band log: with one wave per block (the default) four bands per
+  // wave-level sum through LDS (4 KB), otherwise a butterfly per band
   __shared__ double log_slot[N1LOG ? 2 : 1][kBandLogChunk][4][2];
+  __shared__ double2 log_stage[N1LOG ? 4 : 1][kWave];
   const BandLog<1> blog{log_slot, nullptr, n1log,
                         (size_t)b * (n1_end - n1_begin), n1_end - n1_begin};
+  const BandLog<4> blog4{log_slot, log_stage, n1log,
+                         (size_t)b * (n1_end - n1_begin), n1_end - n1_begin};
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = uniform(threadIdx.x / kWave);
   const int W = blockDim.x / kWave;
@@ -522,7 +527,10 @@ __device__ __forceinline__ void unocc_mixed_block(
       while (qn > 0) drain(min(qn, kWave));
       const double lr = fma(wr, sr[2], wi_ * si[2]) + (acc_r[2] - c2r);
       const double li = fma(wr, si[2], -wi_ * sr[2]) + (acc_i[2] - c2i);
-      blog.put(n1 - n1_begin, lr, li);
+      if (W == 1)  // block-uniform
+        blog4.put(n1 - n1_begin, lr, li);
+      else
+        blog.put(n1 - n1_begin, lr, li);
     } else if (qn >= kWave) {
       drain(kWave);
     }
