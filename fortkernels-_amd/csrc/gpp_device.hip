@@ -207,7 +207,8 @@ __device__ __forceinline__ void unocc_fp64_block(const Prob &p,
   const bool valid = igp >= 0;
 
   double wr[K], wi[K], wi2[K], Ar[K], Ai[K];
-  double wt2[K];  // |wtilde|^2, only for the mask
+  // the mask keeps a term iff |wdiff|^2 > max(limittwo, |wtilde|^2 / limitone)
+  double mthr[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int ig = ig0 + k;
@@ -227,7 +228,7 @@ __device__ __forceinline__ void unocc_fp64_block(const Prob &p,
       Ai[k] = 0.0;
     }
     wi2[k] = wi[k] * wi[k];
-    wt2[k] = wr[k] * wr[k] + wi2[k];
+    mthr[k] = fmax(kLimitTwo, (wr[k] * wr[k] + wi2[k]) / kLimitOne);
   }
 
   // sch_array[iw] = 0.5 * sum; achtemp[iw] += sch_array[iw] * vcoul[igp]
@@ -337,13 +338,12 @@ __device__ __forceinline__ void unocc_fp64_block(const Prob &p,
         if (MASK) {
           // gppkernel.deslippe.09.30.2014.f90:364-366 /
           // BGW-Kernels/gppKer.cpp:406: keep the term only if |wdiff|^2 >
-          // limittwo and |delw|^2 = |wtilde|^2/|wdiff|^2 < limitone.
+          // limittwo and |delw|^2 = |wtilde|^2/|wdiff|^2 < limitone, i.e.
+          // (|wdiff|^2 > 0) |wdiff|^2 > max(limittwo, |wtilde|^2/limitone):
+          // one compare per term against the G-vector's threshold.
 #pragma unroll
           for (int iw = 0; iw < 3; ++iw)
-            r[j][iw] = (d[j][iw] > kLimitTwo &&
-                        wt2[k + j] * r[j][iw] < kLimitOne)
-                           ? r[j][iw]
-                           : 0.0;
+            r[j][iw] = d[j][iw] > mthr[k + j] ? r[j][iw] : 0.0;
         }
         s_r0 = fma(fma(Br[j], dr[j][0], nBiwi[j]), r[j][0], s_r0);
         s_i0 = fma(fma(Bi[j], dr[j][0], Brwi[j]), r[j][0], s_i0);
