@@ -242,9 +242,9 @@ __device__ __forceinline__ void unocc_fp64_block(const Prob &p,
 
   // For K <= 8 the next band's aqsn run and frequencies (wave-uniform, in
   // SGPRs) are loaded a whole band ahead, so no scalar-load latency sits
-  // between the bands; for larger K (or the masked variant's extra live
+  // between the bands; for larger K (or the per-band log's extra live
   // values) the registers do not hold two runs.
-  constexpr bool kAhead = K <= 8 && !MASK && !N1LOG;
+  constexpr bool kAhead = K <= 8 && !N1LOG;
   // per-band log: four bands per wave-level sum (16 KB of LDS per block)
   __shared__ double log_slot[N1LOG ? 2 : 1][kBandLogChunk][4][2];
   __shared__ double2 log_stage[N1LOG ? 4 : 1][N1LOG ? 4 : 1][kWave];
