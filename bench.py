@@ -122,6 +122,14 @@ WINDOW_THRESHOLDS = {"band_resolved": 4e-3, "large": 4e-3, "batch": 4e-3,
 # path name -> Handle.sigma keyword arguments (the window's threshold is set
 # per workload in main())
 PATH_ARGS = {WINDOW: dict(precision="mixed", resonance_threshold=1e-3)}
+# --semantics / --acht-n1-loc: passed to every evaluation (timed and checked)
+SIGMA_EXTRA = {}
+
+
+def path_kw(prec):
+    kw = dict(PATH_ARGS.get(prec, {"precision": prec}))
+    kw.update(SIGMA_EXTRA)
+    return kw
 PEAK_FP32_TFLOPS = 157.3  # FP32 vector peak, the mixed paths' ceiling
 # Band sum first: P = sum_n1 conj(aqsm) aqsn is a complex matrix product,
 # 8 real flops per (n1, my_igp, ig) by the usual count (the per-(my_igp, ig)
@@ -305,16 +313,17 @@ def cpu_leg_check(gppk, h, inp_host: dict, ng_slice: int | None, nb_slice: int,
                    inv_igp_index=sub["inv_igp_index"][:ng_slice], ngpown=ng_slice)
     sub = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v)
            for k, v in sub.items()}
-    ref = orc.sigma(sub, want_achstemp=False)
+    ref = orc.sigma(sub, SIGMA_EXTRA.get("semantics", "gppker"),
+                    want_achstemp=False)
     errs = {}
     for prec in precisions:
-        g = h.sigma(sub, **PATH_ARGS.get(prec, {"precision": prec}))
+        g = h.sigma(sub, **path_kw(prec))
         errs[prec] = max(rel_err(g[k], ref[k]) for k in ("achtemp", "asxtemp"))
     return errs
 
 
 def time_path(h, dev, prec, steps, warmup, stream, gppk):
-    kw = PATH_ARGS.get(prec, {"precision": prec})
+    kw = path_kw(prec)
     call = lambda: h.sigma(dev, memory=gppk.MEM_DEVICE,
                            stream=stream.cuda_stream, **kw)
     import torch
@@ -466,7 +475,7 @@ def run_single(args, name):
         "data": "synthetic",
         "config": {"workload": w["desc"], "number_bands": nb, "nvband": nv,
                    "ncouls": nc, "nodes_per_group": npg, "ngpown": ng,
-                   "terms": terms, "path": best},
+                   "terms": terms, "path": best, **SIGMA_EXTRA},
         "roofline": {"bound": "mfma" if best == FACT else "valu",
                      "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s",
@@ -506,13 +515,14 @@ def run_single(args, name):
         fa = fl / (f["unocc_kernel_ms"] * 1e-3) / 1e12
         # the same evaluation with the per-band log (a second matrix product)
         import torch
+        log_kw = dict(SIGMA_EXTRA, acht_n1_loc=True)
         h.sigma(dev, precision=FACT, memory=gppk.MEM_DEVICE,
-                stream=torch.cuda.current_stream().cuda_stream, acht_n1_loc=True)
+                stream=torch.cuda.current_stream().cuda_stream, **log_kw)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(3):
             h.sigma(dev, precision=FACT, memory=gppk.MEM_DEVICE,
-                    stream=torch.cuda.current_stream().cuda_stream, acht_n1_loc=True)
+                    stream=torch.cuda.current_stream().cuda_stream, **log_kw)
         torch.cuda.synchronize()
         fact_log_ms = (time.perf_counter() - t0) / 3 * 1e3
         log(f"{FACT} with acht_n1_loc: {fact_log_ms:.3f} ms/step")
@@ -823,6 +833,13 @@ def main():
     ap.add_argument("--window", type=float, default=None,
                     help="resonance threshold of the mixed_window path "
                          "(default: per workload, WINDOW_THRESHOLDS)")
+    ap.add_argument("--semantics", choices=("gppker", "strict", "bgw"),
+                    default=None, help="physics variant of every evaluation "
+                    "of a single-problem workload (default: the library's, "
+                    "gppker)")
+    ap.add_argument("--acht-n1-loc", action="store_true",
+                    help="every evaluation of a single-problem workload also "
+                    "returns the per-band log")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (the "
                          "reference kernel for scaling rocprofv3 FETCH_SIZE/"
@@ -849,6 +866,10 @@ def main():
               "(the sum would shard over G' columns with one all-reduce of 12 "
               "doubles); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.semantics:
+        SIGMA_EXTRA["semantics"] = args.semantics
+    if args.acht_n1_loc:
+        SIGMA_EXTRA["acht_n1_loc"] = True
     PATH_ARGS[WINDOW]["resonance_threshold"] = (
         args.window if args.window is not None
         else WINDOW_THRESHOLDS.get(args.workload, 1e-3))

@@ -933,7 +933,7 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   for (int i = 0; i < n; ++i) {
     ach_begin[i] = nrec + nach;
     ach_count[i] =
-        want_ach ? gpp::achstemp_records(prep[i].dp, o->semantics) : 0;
+        want_ach ? gpp::achstemp_batch_records(prep[i].dp, o->semantics) : 0;
     nach += ach_count[i];
   }
 

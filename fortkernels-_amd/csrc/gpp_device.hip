@@ -957,11 +957,18 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
 // time.  The epilogue contracts Q with the weights and conj(aqsm) over the
 // block's G' columns and stores one complex per band into the record of
 // (group, chunk): [record][n1 - n1_begin], the layout launch_reduce_n1 reads.
+// ACH: the same product with I_eps in place of S2 over all bands gives the
+// bgw static remainder (F90 :195-229, the full form of reduce_achstemp,
+// gppKer.cpp:64-120)
+//   achstemp = -sum_{n1, my_igp} 0.5 vcoul[igp] conj(aqsm[n1][igp]) *
+//              sum_ig aqsn[n1][ig] * I_eps[my_igp][ig]
+// and each block adds its bands (in order) into one kSlots record of
+// -(its share), records numbered over the grid, for launch_reduce.
 constexpr int kLogM = 64;        // bands per block
 constexpr int kLogStage = 8;     // G-vectors per LDS stage
 constexpr int kLogChunk = 2048;  // G-vectors per block
 
-template <bool MASK>
+template <bool MASK, bool ACH = false>
 __global__ __launch_bounds__(kGemmThreads) __attribute__((
     amdgpu_waves_per_eu(kGemmWaves))) void n1log_gemm_kernel(
     DevProblem p, int n1_begin, int n1_end, int grp_lo,
@@ -1014,7 +1021,7 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
       gin[c] = ig < k_end;
       const int igc = min(ig, NC - 1);
       ga[c] = arow[igc];
-      gw[c] = wrow[igc];
+      if (!ACH) gw[c] = wrow[igc];
       ge[c] = erow[igc];
     }
   };
@@ -1024,6 +1031,10 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
       const bool live = gin[c] && igp_s >= 0;
       As[buf][sr][sc + c] =
           (gin[c] && a_row) ? ga[c] : make_double2(0.0, 0.0);
+      if (ACH) {  // I_eps itself (0 for inert slots)
+        Bs[buf][sr][sc + c] = live ? ge[c] : make_double2(0.0, 0.0);
+        continue;
+      }
       // S2 = A conj(wdiff_2) r_2, A = wtilde I_eps (0 for inert slots)
       const double2 w = gw[c], e = ge[c];
       const double wr = live ? w.x : 0.0, wi = live ? w.y : 0.0;
@@ -1124,12 +1135,34 @@ __global__ __launch_bounds__(kGemmThreads) __attribute__((
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int i = wm + 16 * mb + lk + 4 * e;
-        if (i0 + i < nbu) {
+        const bool in = i0 + i < nbu;
+        const double br = v[mb][e][0] + red[1][i][0];
+        const double bi = v[mb][e][1] + red[1][i][1];
+        if (ACH) {
+          red[0][i][0] = in ? br : 0.0;
+          red[0][i][1] = in ? bi : 0.0;
+        } else if (in) {
           double *o = n1log + 2 * (rec * nbu + i0 + i);
-          o[0] = v[mb][e][0] + red[1][i][0];
-          o[1] = v[mb][e][1] + red[1][i][1];
+          o[0] = br;
+          o[1] = bi;
         }
       }
+  }
+  if (ACH) {
+    __syncthreads();
+    double *o = n1log + (((size_t)blockIdx.z * gridDim.y + blockIdx.y) *
+                             gridDim.x + blockIdx.x) * kSlots;
+    if (threadIdx.x == 0) {
+      double sr = 0.0, si = 0.0;
+      for (int i = 0; i < kLogM; ++i) {
+        sr += red[0][i][0];
+        si += red[0][i][1];
+      }
+      o[0] = -sr;
+      o[1] = -si;
+    } else if (threadIdx.x >= 2 && threadIdx.x < kSlots) {
+      o[threadIdx.x] = 0.0;
+    }
   }
 }
 
@@ -1292,12 +1325,6 @@ __device__ __forceinline__ void achstemp_bgw_block(const DevProblem &p, int blk,
   double v[2] = {w * vr, w * vi};
   block_store<2>(v, rec);
   if (threadIdx.x >= 2 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
-}
-
-__global__ __launch_bounds__(kAchThreads) void achstemp_bgw_kernel(
-    DevProblem p, double *__restrict__ partials, int rec_base) {
-  achstemp_bgw_block(p, blockIdx.x,
-                     partials + (size_t)(rec_base + blockIdx.x) * kSlots);
 }
 
 __host__ __device__ inline int achstemp_nrec(const DevProblem &p, int sem) {
@@ -1648,7 +1675,23 @@ hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,
   return hipGetLastError();
 }
 
+// The bgw static remainder of one problem runs on the matrix cores
+// (n1log_gemm_kernel<false, true>): one record per block of that grid.
+static dim3 achstemp_gemm_grid(const DevProblem &p) {
+  if (p.NB <= 0 || p.NG <= 0 || p.NC <= 0) return dim3(0, 0, 0);
+  return dim3((p.NB + kLogM - 1) / kLogM, unocc_groups(p),
+              (p.NC + kLogChunk - 1) / kLogChunk);
+}
+
 int achstemp_records(const DevProblem &p, int sem) {
+  if (sem == kSemBgw) {
+    const dim3 g = achstemp_gemm_grid(p);
+    return (int)(g.x * g.y * g.z);
+  }
+  return achstemp_nrec(p, sem);
+}
+
+int achstemp_batch_records(const DevProblem &p, int sem) {
   return achstemp_nrec(p, sem);
 }
 
@@ -1693,9 +1736,13 @@ hipError_t launch_achstemp(const DevProblem &p, int sem, double *partials,
                            hipStream_t s) {
   const int n = achstemp_records(p, sem);
   if (n == 0) return hipSuccess;
-  if (sem == kSemBgw)
-    achstemp_bgw_kernel<<<n, kAchThreads, 0, s>>>(p, partials, 0);
-  else
+  if (sem == kSemBgw) {
+    const dim3 g = achstemp_gemm_grid(p);
+    if (g.x > 65535u * 1024u || g.y > 65535u || g.z > 65535u)
+      return hipErrorInvalidValue;
+    n1log_gemm_kernel<false, true><<<g, kGemmThreads, 0, s>>>(p, 0, p.NB, 0,
+                                                             partials);
+  } else
     achstemp_gppker_kernel<<<n, kAchThreads, 0, s>>>(p, partials, 0);
   return hipGetLastError();
 }
