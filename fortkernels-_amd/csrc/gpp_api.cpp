@@ -929,11 +929,16 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
   int n_u = 0;
   while (n_u < (int)items.size() && items[n_u].kind == 0) ++n_u;
   const int n_o = (int)items.size() - n_u;
+  // The bgw static remainder runs per problem on the matrix cores (one
+  // launch each, records in that kernel's layout); the gppker one, a term
+  // per G' row, in one launch over the batch.
+  const bool ach_gemm = o->semantics == GPP_SEM_BGW;
   int nach = 0;
   for (int i = 0; i < n; ++i) {
     ach_begin[i] = nrec + nach;
-    ach_count[i] =
-        want_ach ? gpp::achstemp_batch_records(prep[i].dp, o->semantics) : 0;
+    ach_count[i] = !want_ach ? 0
+                   : ach_gemm ? gpp::achstemp_records(prep[i].dp, o->semantics)
+                              : gpp::achstemp_batch_records(prep[i].dp, o->semantics);
     nach += ach_count[i];
   }
 
@@ -1028,9 +1033,17 @@ int gpp_sigma_batch(gpp_handle h, const gpp_problem *p, int n,
         crc = set_err(h, GPP_EDEVICE, "batch launch: %s", hipGetErrorString(e));
     }
     if (crc == GPP_OK && want_ach) {
-      hipError_t e = gpp::launch_achstemp_batch(
-          (const DevProblem *)(mb + off_tab), (const int *)(mb + off_ab), n,
-          max_ach, o->semantics, part, s);
+      hipError_t e = hipSuccess;
+      if (ach_gemm) {
+        for (int i = 0; i < n && e == hipSuccess; ++i)
+          if (ach_count[i] > 0)
+            e = gpp::launch_achstemp(prep[i].dp, o->semantics,
+                                     part + (size_t)ach_begin[i] * gpp::kSlots, s);
+      } else {
+        e = gpp::launch_achstemp_batch(
+            (const DevProblem *)(mb + off_tab), (const int *)(mb + off_ab), n,
+            max_ach, o->semantics, part, s);
+      }
       if (e != hipSuccess)
         crc = set_err(h, GPP_EDEVICE, "achstemp: %s", hipGetErrorString(e));
     }
