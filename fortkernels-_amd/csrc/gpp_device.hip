@@ -1291,61 +1291,24 @@ __global__ __launch_bounds__(kAchThreads) void achstemp_gppker_kernel(
                         partials + (size_t)(rec_base + blockIdx.x) * kSlots);
 }
 
-// Static remainder, F90 form (gppkernel.deslippe.09.30.2014.f90:195-229):
-//   -0.5 * vcoul[igp] * conj(aqsm[n1][igp]) * sum_ig aqsn[n1][ig] *
-//   I_eps[my_igp][ig]
-// One block per (my_igp, band chunk); lanes over ig.
-constexpr int kAchBands = 8;
-__device__ __forceinline__ void achstemp_bgw_block(const DevProblem &p, int blk,
-                                                   double *__restrict__ rec) {
-  const int nbc = (p.NB + kAchBands - 1) / kAchBands;
-  const int my_igp = blk / nbc;
-  const int n1b = (blk % nbc) * kAchBands;
-  const int n1e = min(p.NB, n1b + kAchBands);
-  const int igp = p.igp_of[my_igp];
-  if (igp < 0) {
-    if (threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
-    return;
-  }
-  double vr = 0.0, vi = 0.0;
-  for (int n1 = n1b; n1 < n1e; ++n1) {
-    double sr = 0.0, si = 0.0;
-    for (int ig = threadIdx.x; ig < p.NC; ig += kAchThreads) {
-      const double2 an = p.aqsn[(size_t)n1 * p.NC + ig];
-      const double2 e = p.ieps[(size_t)my_igp * p.NC + ig];
-      sr += an.x * e.x - an.y * e.y;
-      si += an.x * e.y + an.y * e.x;
-    }
-    const double2 am = p.aqsm[(size_t)n1 * p.NC + igp];
-    // -(conj(am) * s)
-    vr -= am.x * sr + am.y * si;
-    vi -= am.x * si - am.y * sr;
-  }
-  const double w = 0.5 * p.vcoul[igp];
-  double v[2] = {w * vr, w * vi};
-  block_store<2>(v, rec);
-  if (threadIdx.x >= 2 && threadIdx.x < kSlots) rec[threadIdx.x] = 0.0;
-}
-
-__host__ __device__ inline int achstemp_nrec(const DevProblem &p, int sem) {
+// The F90 form of the static remainder (gppkernel.deslippe.09.30.2014.f90:
+// 195-229, every G-vector) runs on the matrix cores: n1log_gemm_kernel<false,
+// true>, launched per problem by launch_achstemp (batches included).
+__host__ __device__ inline int achstemp_nrec(const DevProblem &p) {
   if (p.NG <= 0 || p.NC <= 0 || p.NB <= 0) return 0;
-  if (sem == kSemBgw) return p.NG * ((p.NB + kAchBands - 1) / kAchBands);
   return p.NG;
 }
 
-// All problems of a batch: problem blockIdx.y, record rec_begin[prob] +
-// blockIdx.x (blocks past a problem's record count return at once).
-template <int SEM>
+// All problems of a batch, gppKer.cpp form: problem blockIdx.y, record
+// rec_begin[prob] + blockIdx.x (blocks past a problem's record count return
+// at once).
 __global__ __launch_bounds__(kAchThreads) void achstemp_batch_kernel(
     const DevProblem *__restrict__ probs, const int *__restrict__ rec_begin,
     double *__restrict__ partials) {
   const DevProblem p = probs[blockIdx.y];
-  if ((int)blockIdx.x >= achstemp_nrec(p, SEM)) return;  // block-uniform
+  if ((int)blockIdx.x >= achstemp_nrec(p)) return;  // block-uniform
   double *rec = partials + (size_t)(rec_begin[blockIdx.y] + blockIdx.x) * kSlots;
-  if (SEM == kSemBgw)
-    achstemp_bgw_block(p, blockIdx.x, rec);
-  else
-    achstemp_gppker_block(p, blockIdx.x, rec);
+  achstemp_gppker_block(p, blockIdx.x, rec);
 }
 
 // ---------------------------------------------------------------------------
@@ -1688,11 +1651,11 @@ int achstemp_records(const DevProblem &p, int sem) {
     const dim3 g = achstemp_gemm_grid(p);
     return (int)(g.x * g.y * g.z);
   }
-  return achstemp_nrec(p, sem);
+  return achstemp_nrec(p);
 }
 
 int achstemp_batch_records(const DevProblem &p, int sem) {
-  return achstemp_nrec(p, sem);
+  return sem == kSemBgw ? 0 : achstemp_nrec(p);
 }
 
 hipError_t launch_prepare_batch(const DevProblem *probs, const PrepItem *preps,
@@ -1722,13 +1685,11 @@ hipError_t launch_achstemp_batch(const DevProblem *probs, const int *rec_begin,
                                  double *partials, hipStream_t s) {
   if (n <= 0 || max_rec <= 0) return hipSuccess;
   if (n > 65535) return hipErrorInvalidValue;
+  // bgw batches launch the matrix-core form per problem (launch_achstemp)
+  if (sem == kSemBgw) return hipErrorInvalidValue;
   const dim3 grid((unsigned)max_rec, n);
-  if (sem == kSemBgw)
-    achstemp_batch_kernel<kSemBgw><<<grid, kAchThreads, 0, s>>>(probs, rec_begin,
-                                                                 partials);
-  else
-    achstemp_batch_kernel<kSemGppker><<<grid, kAchThreads, 0, s>>>(
-        probs, rec_begin, partials);
+  achstemp_batch_kernel<<<grid, kAchThreads, 0, s>>>(probs, rec_begin,
+                                                     partials);
   return hipGetLastError();
 }
 

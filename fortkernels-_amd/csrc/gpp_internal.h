@@ -137,7 +137,8 @@ hipError_t launch_reduce_n1(const double *src_u, int nrec_u, int u_begin,
                             int nb_u, const double *src_o, int nrec_o,
                             int nb_o, int NB, double *out, hipStream_t s);
 int achstemp_records(const DevProblem &p, int sem);
-// Records of the batched form (achstemp_batch_kernel's layout).
+// Records of the batched gppKer.cpp form (achstemp_batch_kernel's layout;
+// 0 for bgw, whose batches launch launch_achstemp per problem).
 int achstemp_batch_records(const DevProblem &p, int sem);
 // Prepass, aqsm gather and (mixed) FP32 copy of aqsn for problems
 // [0, n) of a device table, one launch each (problem = blockIdx.y);
